@@ -1,0 +1,102 @@
+"""NumPy front end of the CPU checkers (TEST INFRASTRUCTURE ONLY: used by tests/, the smoke check and bench.py's
+cpu_baseline leg, never by the product).
+
+    Checker("oracle")     this repository's scalar C restatement (oracle/liboracle.so, built by `make -C oracle`)
+    Checker("ref")        the reference C library compiled from BodeTobias/matfun's own sources
+                          (oracle/_ref/libmatfun_ref.so; None when it was not built)
+
+Both expose smf(kind, alpha, v, order), smf_matrix(kind, alpha, A, order), gmf(kind, alpha, A, order) returning
+(rc, F, J, H) with J / H None below the order, in the layouts of c/include/matfun/matfun.h.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+PATHS = {"oracle": os.path.join(HERE, "liboracle.so"), "ref": os.path.join(HERE, "_ref", "libmatfun_ref.so")}
+KINDS = {"exp": 0, "log": 1, "pow": 2, "sqrt": 3}
+
+
+def available(which):
+    return os.path.exists(PATHS[which])
+
+
+class Checker:
+    def __init__(self, which="oracle"):
+        path = PATHS[which]
+        if not os.path.exists(path):
+            raise FileNotFoundError(f"{path} not built (make -C oracle)")
+        self.which = which
+        self.lib = ctypes.CDLL(path)
+        pre = "oracle_" if which == "oracle" else ""
+        args = [ctypes.c_int, ctypes.c_double, ctypes.c_int64] + [ctypes.c_void_p] * 4
+        self._smf = getattr(self.lib, pre + "smf_eval")
+        self._smfm = getattr(self.lib, pre + "smf_eval_matrix")
+        self._gmf = getattr(self.lib, pre + "gmf_eval")
+        for f in (self._smf, self._smfm, self._gmf):
+            f.argtypes, f.restype = args, ctypes.c_int
+        if which == "oracle":
+            aargs = [ctypes.c_int, ctypes.c_void_p, ctypes.c_int64] + [ctypes.c_void_p] * 4
+            self._smfa, self._gmfa = self.lib.oracle_smf_eval_matrix_alpha, self.lib.oracle_gmf_eval_alpha
+            for f in (self._smfa, self._gmfa):
+                f.argtypes, f.restype = aargs, ctypes.c_int
+            self.lib.oracle_set_threads.argtypes = [ctypes.c_int]
+            self.lib.oracle_gmf_terms.argtypes = [ctypes.c_int, ctypes.c_double, ctypes.c_int, ctypes.c_void_p]
+            self.lib.oracle_gmf_terms.restype = ctypes.c_int
+        else:
+            self.lib.smf_simd_name.restype = ctypes.c_char_p
+
+    def name(self):
+        if self.which == "ref":
+            return "reference C (" + self.lib.smf_simd_name().decode() + ")"
+        return "oracle (scalar C restatement)"
+
+    def set_threads(self, n):
+        if self.which == "oracle":
+            self.lib.oracle_set_threads(int(n))
+
+    @staticmethod
+    def _kind(kind):
+        return KINDS[kind] if isinstance(kind, str) else int(kind)
+
+    def _run(self, fn, kind, alpha, X, nin, nout, order):
+        X = np.ascontiguousarray(X, dtype=np.float64).reshape(-1, nin)
+        n = X.shape[0]
+        F = np.empty((n, nout))
+        J = np.empty((n, nout * nout)) if order >= 1 else None
+        H = np.empty((n, nout ** 3)) if order >= 2 else None
+        p = lambda a: None if a is None else a.ctypes.data
+        rc = fn(self._kind(kind), float(alpha), n, X.ctypes.data, F.ctypes.data, p(J), p(H))
+        return rc, F, J, H
+
+    def smf(self, kind, alpha, v, order=2):
+        return self._run(self._smf, kind, alpha, v, 6, 6, order)
+
+    def smf_matrix(self, kind, alpha, A, order=2):
+        return self._run(self._smfm, kind, alpha, A, 9, 9, order)
+
+    def gmf(self, kind, alpha, A, order=2):
+        return self._run(self._gmf, kind, alpha, A, 9, 9, order)
+
+    def _run_alpha(self, fn, alphas, X, order):
+        X = np.ascontiguousarray(X, dtype=np.float64).reshape(-1, 9)
+        alphas = np.ascontiguousarray(alphas, dtype=np.float64)
+        n = X.shape[0]
+        assert alphas.shape == (n,)
+        F = np.empty((n, 9))
+        J = np.empty((n, 81)) if order >= 1 else None
+        H = np.empty((n, 729)) if order >= 2 else None
+        p = lambda a: None if a is None else a.ctypes.data
+        rc = fn(2, alphas.ctypes.data, n, X.ctypes.data, F.ctypes.data, p(J), p(H))
+        return rc, F, J, H
+
+    def smf_matrix_alpha(self, alphas, A, order=2):
+        return self._run_alpha(self._smfa, alphas, A, order)
+
+    def gmf_alpha(self, alphas, A, order=2):
+        return self._run_alpha(self._gmfa, alphas, A, order)
+
+    def gmf_terms(self, kind, alpha, order, A):
+        A = np.ascontiguousarray(A, dtype=np.float64).reshape(9)
+        return self.lib.oracle_gmf_terms(self._kind(kind), float(alpha), int(order), A.ctypes.data)
