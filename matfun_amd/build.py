@@ -1,0 +1,51 @@
+"""Builds the HIP library libmatfun_hip.so (gfx950) in-tree from matfun_amd/csrc with hipcc.
+
+    python -m matfun_amd.build            (or build() from Python)
+
+Flags: -O3, C++17, position independent, packed-FP32 instruction selection disabled
+(-Xclang -target-feature -Xclang -packed-fp32-ops), FP contraction as hipcc's default (FMA).
+"""
+import os
+import shutil
+import subprocess
+import sys
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+CSRC = os.path.join(HERE, "csrc")
+LIB = os.path.join(HERE, "libmatfun_hip.so")
+SOURCES = ["matfun_hip.hip", "smf_kernels.hip", "gmf_kernels.hip"]
+ARCH = os.environ.get("MATFUN_HIP_ARCH", "gfx950")
+
+
+def hipcc():
+    for c in (shutil.which("hipcc"), "/opt/rocm/bin/hipcc"):
+        if c and os.path.exists(c):
+            return c
+    raise RuntimeError("hipcc not found")
+
+
+def flags(extra=()):
+    return ["--offload-arch=" + ARCH, "-O3", "-std=c++17", "-fPIC", "-Xclang", "-target-feature", "-Xclang",
+            "-packed-fp32-ops", "-I" + CSRC, *extra]
+
+
+def build(verbose=False, extra=()):
+    objs = []
+    for src in SOURCES:
+        obj = os.path.join(CSRC, os.path.splitext(src)[0] + ".o")
+        cmd = [hipcc(), *flags(extra), "-c", os.path.join(CSRC, src), "-o", obj]
+        if verbose:
+            print(" ".join(cmd), flush=True)
+        subprocess.run(cmd, check=True)
+        objs.append(obj)
+    tmp = LIB + ".tmp"
+    cmd = [hipcc(), "--offload-arch=" + ARCH, "-shared", "-fPIC", *objs, "-o", tmp]
+    subprocess.run(cmd, check=True)
+    os.replace(tmp, LIB)
+    for o in objs:
+        os.remove(o)
+    return LIB
+
+
+if __name__ == "__main__":
+    print(build(verbose=True, extra=sys.argv[1:]))

@@ -1,0 +1,101 @@
+// matfun_hip.hip -- the C ABI (include/matfun_hip.h): argument checks, exponent normalisation, host-built series
+// tables, batch splitting, launch-error reporting.  Semantics of the reference's CUDA API
+// (cuda/src/matfun_cuda.cu:119-141 launch, :169-190 glaunch, :194-207 the exported functions).
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "../../include/matfun_hip.h"
+#include "gmf_device.hpp"
+#include "smf_device.hpp"
+#include "smf_launch.hpp"
+
+#ifndef MATFUN_HIP_VERSION
+#define MATFUN_HIP_VERSION "0.1.3-hip.1"
+#endif
+
+namespace {
+
+// Taylor coefficients of the second divided differences, f^(k+2)(m) / (k+2)! without the m-dependent factor:
+// exp 1/(k+2)!, log (-1)^(k+1)/(k+2), powers binom(al, k+2)  (smf_kernel.hpp:135-146)
+void smf_coefs(int kind, double al, mf::SmfCoef& c) {
+  if (kind == mf::EXP) {
+    double f = 0.5;
+    for (int k = 0; k < mf::SMF_NTERMS; k++) { c.c[k] = f; f /= (k + 3.0); }
+  } else if (kind == mf::LOG) {
+    for (int k = 0; k < mf::SMF_NTERMS; k++) c.c[k] = ((k & 1) ? 1.0 : -1.0) / (k + 2.0);
+  } else {
+    double b = al * (al - 1.0) / 2.0;
+    for (int k = 0; k < mf::SMF_NTERMS; k++) { c.c[k] = b; b *= (al - k - 2.0) / (k + 3.0); }
+  }
+}
+
+// b_j of f(c (1 + u)): exp 1/j!, log (-1)^(j+1)/j, pow binom(al, j)  (gmf_kernel.hpp:48-53)
+void gmf_coefs(int kind, double al, mf::GmfCoef& c) {
+  c.b[0] = 1.0;
+  for (int j = 1; j < mf::GMF_KMAX; j++)
+    c.b[j] = kind == mf::EXP ? c.b[j - 1] / j : (kind == mf::LOG ? ((j & 1) ? 1.0 : -1.0) / j : c.b[j - 1] * (al - j + 1) / j);
+}
+
+bool valid(int kind, double alpha, int64_t n, const double* J, const double* H) {
+  return !(kind < mf::EXP || kind > mf::SQRT || n < 0 || (H && !J) || (kind == mf::POW && !(alpha > 0)));
+}
+
+// the reference's grid limit: more than 2^31 - 1 blocks of 32 (smf) / 64 (gmf) matrices is an error, reported before
+// any launch (matfun_cuda.cu:126-131)
+constexpr int64_t SMF_MAX_N = (int64_t)INT32_MAX * 32, GMF_MAX_N = (int64_t)INT32_MAX * 64;
+
+int launched() {
+  const hipError_t e = hipGetLastError();
+  return e == hipSuccess ? 0 : -2;
+}
+
+int smf_call(int kind, double alpha, int64_t n, const double* x, bool mat, double* F, double* J, double* H,
+             void* stream) {
+  if (!valid(kind, alpha, n, J, H)) return -1;
+  if (kind != mf::POW) alpha = kind == mf::SQRT ? 0.5 : 0.0;
+  if (n == 0) return 0;
+  if (n > SMF_MAX_N) return -2;
+  mf::SmfCoef c2;
+  smf_coefs(kind, alpha, c2);
+  const int nin = mat ? 9 : 6, nf = mat ? 9 : 6;
+  for (int64_t lo = 0; lo < n; lo += mf::MAX_PER_LAUNCH) {
+    const int64_t m = n - lo < mf::MAX_PER_LAUNCH ? n - lo : mf::MAX_PER_LAUNCH;
+    mf::smf_launch((hipStream_t)stream, kind, mat, m, x + nin * lo, F + nf * lo, J ? J + nf * nf * lo : nullptr,
+                   H ? H + nf * nf * nf * lo : nullptr, alpha, c2);
+    if (launched()) return -2;
+  }
+  return 0;
+}
+
+}  // namespace
+
+extern "C" int smf_hip_eval(int kind, double alpha, int64_t n, const double* v, double* F, double* J, double* H,
+                            void* stream) {
+  return smf_call(kind, alpha, n, v, false, F, J, H, stream);
+}
+
+extern "C" int smf_hip_eval_matrix(int kind, double alpha, int64_t n, const double* A, double* F, double* J, double* H,
+                                   void* stream) {
+  return smf_call(kind, alpha, n, A, true, F, J, H, stream);
+}
+
+extern "C" int gmf_hip_eval(int kind, double alpha, int64_t n, const double* A, double* F, double* J, double* H,
+                            void* stream) {
+  if (!valid(kind, alpha, n, J, H)) return -1;
+  if (kind == mf::SQRT) kind = mf::POW, alpha = 0.5;
+  if (kind != mf::POW) alpha = 0.0;
+  if (n == 0) return 0;
+  if (n > GMF_MAX_N) return -2;
+  mf::GmfCoef bt;
+  gmf_coefs(kind, alpha, bt);
+  for (int64_t lo = 0; lo < n; lo += mf::MAX_PER_LAUNCH) {
+    const int64_t m = n - lo < mf::MAX_PER_LAUNCH ? n - lo : mf::MAX_PER_LAUNCH;
+    mf::gmf_launch((hipStream_t)stream, kind, m, A + 9 * lo, F + 9 * lo, J ? J + 81 * lo : nullptr,
+                   H ? H + 729 * lo : nullptr, alpha, bt);
+    if (launched()) return -2;
+  }
+  return 0;
+}
+
+extern "C" const char* smf_hip_version(void) { return MATFUN_HIP_VERSION; }
+extern "C" const char* smf_hip_target(void) { return "gfx950"; }

@@ -1,0 +1,19 @@
+// smf_launch.hpp -- host-side entry points of the kernel files (used by the C ABI in matfun_hip.hip).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace mf {
+
+struct SmfCoef;
+struct GmfCoef;
+
+// one launch over n matrices (n <= MAX_PER_LAUNCH); kind already normalised; J / H select the order
+void smf_launch(hipStream_t st, int kind, bool mat, int64_t n, const double* x, double* F, double* J, double* H,
+                double al, const SmfCoef& c2);
+void gmf_launch(hipStream_t st, int kind, int64_t n, const double* A, double* F, double* J, double* H, double al,
+                const GmfCoef& bt);
+
+constexpr int64_t MAX_PER_LAUNCH = (int64_t)1 << 30;   // matrices per grid (work-items stay below 2^32)
+
+}  // namespace mf

@@ -1,0 +1,160 @@
+"""gmf_hip_eval on the GPU against the CPU oracle (bitwise equal to the reference C kernel) and the reference's
+golden vectors.  Tolerances: FP64 per-matrix normwise relative error <= 1e-12 (F, J, H separately) on the seeded
+general workload (similarity transforms, complex pairs, shear, 4 I + N(0,1)); max-norm batch difference <= 1e-13 on
+the reference GPU test's sample (cuda/tests/test_cuda_api.cu:81-112); golden cases at their own tolerances."""
+import os
+
+import numpy as np
+import pytest
+
+from oracle.pyoracle import Checker
+from matfun_amd import workloads as W
+from tests.parity import maxrel, normwise
+
+pytestmark = pytest.mark.gpu
+GOLD = os.path.join(os.path.dirname(__file__), "golden")
+KINDS = [("exp", 0.0), ("log", 0.0), ("sqrt", 0.0), ("pow", 0.3), ("pow", 1.7)]
+TOL = 1e-12
+
+
+@pytest.fixture(scope="module")
+def oracle():
+    return Checker("oracle")
+
+
+@pytest.fixture(scope="module")
+def gen():
+    return W.general(3000, seed=202)
+
+
+@pytest.mark.parametrize("kind,alpha", KINDS)
+@pytest.mark.parametrize("order", [0, 1, 2])
+def test_general_vs_oracle(oracle, gen, kind, alpha, order):
+    from tests.gpu_util import call
+    rc, F, J, H = call("general", kind, alpha, gen, order)
+    orc, Fo, Jo, Ho = oracle.gmf(kind, alpha, gen, order)
+    assert rc == 0 and orc == 0
+    for name, x, y in (("F", F, Fo), ("J", J, Jo), ("H", H, Ho))[:order + 1]:
+        e = normwise(x, y)
+        assert e.max() <= TOL, f"{kind} {alpha} {name}: max {e.max():.2e} at {int(np.argmax(e))}, p99.9 {np.quantile(e, 0.999):.2e}"
+
+
+def test_golden_vectors():
+    """the reference's 204 gmf golden cases through the GPU at their per-case tolerances (1e-12, 1e-11, 1e-9)"""
+    from tests.gpu_util import call
+    g = np.load(os.path.join(GOLD, "cases_gmf.npz"))
+    for kind in (0, 1, 2, 3):
+        for alpha in np.unique(g["alpha"][g["kind"] == kind]):
+            sel = np.nonzero((g["kind"] == kind) & (g["alpha"] == alpha))[0]
+            rc, F, J, H = call("general", int(kind), float(alpha), g["A"][sel], 2)
+            assert rc == 0
+            for j, i in enumerate(sel):
+                for name, x, ref in (("F", F[j], g["F"][i]), ("J", J[j], g["J"][i]), ("H", H[j], g["H"][i])):
+                    e = np.max(np.abs(x - ref)) / np.max(np.abs(ref))
+                    assert e <= g["tol"][i], (str(g["label"][i]), int(kind), float(alpha), name, e)
+
+
+def test_reference_gpu_api_sample(oracle):
+    """1.5 I + 0.3 N(0,1) with a shear (3), a rotation block (4) and eigenvalue -1 (5) -> NaN for log/sqrt/pow only
+    there (cuda/tests/test_cuda_api.cu:81-112); batch max-norm relative difference <= 1e-13"""
+    from tests.gpu_util import call
+    rng = np.random.default_rng(8)
+    n = 1000
+    G = 0.3 * rng.standard_normal((n, 9)) + np.eye(3).ravel() * 1.5
+    G[3] = [1, 0.7, 0, 0, 1, 0, 0, 0, 1]
+    G[4] = [0.6, -0.8, 0, 0.8, 0.6, 0, 0, 0, 1.2]
+    G[5] = [-1, 0, 0, 0, 2, 0, 0, 0, 3]
+    for kind, alpha in KINDS:
+        for order in (0, 1, 2):
+            rc, F, J, H = call("general", kind, alpha, G, order)
+            orc, Fo, Jo, Ho = oracle.gmf(kind, alpha, G, order)
+            assert rc == 0 and orc == 0
+            assert np.isnan(F[5]).all() == (kind != "exp") and np.isnan(Fo[5]).all() == (kind != "exp")
+            e = maxrel(F, Fo)
+            if order >= 1:
+                e = max(e, maxrel(J, Jo))
+            if order >= 2:
+                e = max(e, maxrel(H, Ho))
+            assert e <= 1e-13, (kind, alpha, order, e)
+
+
+def test_closed_forms():
+    """exp(N) = I + N, log(I + N) = N, sqrt(4 I + N) = 2 I + N / 4 (defective), exp(skew) = rotation, log(rotation) =
+    skew, sqrt(rotation) = half rotation (c/tests/test_c.c:184-201, tol 1e-15)"""
+    from tests.gpu_util import call
+    th = 0.9
+    c, sn, ch, sh = np.cos(th), np.sin(th), np.cos(th / 2), np.sin(th / 2)
+    Nil = np.array([0, 0.7, 0, 0, 0, 0, 0, 0, 0.0])
+    IN = np.array([1, 0.7, 0, 0, 1, 0, 0, 0, 1.0])
+    I4N = np.array([4, 0.7, 0, 0, 4, 0, 0, 0, 4.0])
+    K = np.array([0, -th, 0, th, 0, 0, 0, 0, 0])
+    R = np.array([c, -sn, 0, sn, c, 0, 0, 0, 1])
+    Rh = np.array([ch, -sh, 0, sh, ch, 0, 0, 0, 1])
+    S2 = np.array([2, 0.175, 0, 0, 2, 0, 0, 0, 2.0])
+    for kind, A, ref in (("exp", Nil, IN), ("log", IN, Nil), ("sqrt", I4N, S2), ("exp", K, R), ("log", R, K),
+                         ("sqrt", R, Rh)):
+        rc, F, _, _ = call("general", kind, 0.0, A[None], 0)
+        assert rc == 0 and maxrel(F[0], ref) <= 1e-15, (kind, maxrel(F[0], ref))
+
+
+def test_derivatives_vs_central_differences():
+    """J and H against central differences of F and J (c/tests/test_c.c:203-228: h = 1e-6, tol 1e-8 / 1e-7), a random
+    and a nearly defective matrix"""
+    from tests.gpu_util import call
+    Ag = np.array([[1.6, 0.3, -0.2, -0.4, 1.1, 0.5, 0.2, -0.3, 1.4], [1.2, 0.8, 0.1, 0, 1.2, 0.5, 1e-9, 0, 1.2]])
+    h = 1e-6
+    for kind, alpha in (("exp", 0.0), ("log", 0.0), ("sqrt", 0.0), ("pow", 1.7)):
+        for A in Ag:
+            P = np.repeat(A[None], 19, axis=0)
+            for p in range(9):
+                P[1 + 2 * p, p] += h
+                P[2 + 2 * p, p] -= h
+            rc, F, J, H = call("general", kind, alpha, P, 2)
+            assert rc == 0
+            Jfd = np.stack([(F[1 + 2 * p] - F[2 + 2 * p]) / (2 * h) for p in range(9)], axis=1)
+            Hfd = np.stack([(J[1 + 2 * p] - J[2 + 2 * p]) / (2 * h) for p in range(9)], axis=1)
+            assert maxrel(J[0].reshape(9, 9), Jfd) <= 1e-8, kind
+            assert maxrel(H[0].reshape(81, 9), Hfd) <= 1e-7, kind
+
+
+def test_batch_equals_single_and_nan_isolation(oracle):
+    """a batch of 37 (lanes with different groups / refinements) == one matrix at a time, and the NaN of a real
+    eigenvalue < 0 stays in its matrix (c/tests/test_c.c:230-255)"""
+    from tests.gpu_util import call
+    n = 37
+    Ab = np.empty((n, 9))
+    for i in range(n):
+        t = 0.37 * i
+        Ab[i] = [1.5 + np.sin(t), 0.3 * np.cos(2 * t), 0.2, -0.4 * np.sin(t), 1.2, 0.5 * np.cos(t), 0.1 * (i % 3), -0.3,
+                 1.4 + 0.2 * np.cos(3 * t)]
+    Ab[11] = [-1, 0, 0, 0, 2, 0, 0, 0, 3]
+    rc, Fg, Jg, Hg = call("general", "log", 0.0, Ab, 2)
+    assert rc == 0
+    for i in range(n):
+        _, F1, J1, H1 = call("general", "log", 0.0, Ab[i:i + 1], 2)
+        if i == 11:
+            assert np.isnan(Fg[i]).all() and np.isnan(F1).all()
+            continue
+        assert max(maxrel(Fg[i], F1[0]), maxrel(Jg[i], J1[0]), maxrel(Hg[i], H1[0])) <= 1e-14
+
+
+def test_h_symmetry_cd_ef():
+    """H_abcdef = H_abefcd (python/tests/test_gmf.py:98)"""
+    from tests.gpu_util import call
+    G = W.general(256, seed=3)
+    _, _, _, H = call("general", "log", 0.0, G, 2)
+    H6 = H.reshape(-1, 3, 3, 3, 3, 3, 3)
+    ok = ~np.isnan(H6).any(axis=(1, 2, 3, 4, 5, 6))
+    d = np.abs(H6 - H6.transpose(0, 1, 2, 5, 6, 3, 4))[ok].max()
+    assert d <= 1e-13 * np.abs(H6[ok]).max()
+
+
+@pytest.mark.parametrize("n", [1, 2, 27, 63, 64, 65, 1000])
+def test_batch_sizes(oracle, n):
+    from tests.gpu_util import call
+    G = W.general(n, seed=17 + n)
+    rc, F, J, H = call("general", "sqrt", 0.0, G, 2)
+    _, Fo, Jo, Ho = oracle.gmf("sqrt", 0.0, G, 2)
+    assert rc == 0
+    for x, y in ((F, Fo), (J, Jo), (H, Ho)):
+        assert normwise(x, y).max() <= TOL

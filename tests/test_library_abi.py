@@ -1,0 +1,61 @@
+"""The HIP library loads and exports every entry point declared in include/matfun_hip.h (no GPU needed: only the
+symbols are looked up, nothing is called that touches a device)."""
+import ctypes
+import os
+import re
+
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+LIB = os.path.join(ROOT, "matfun_amd", "libmatfun_hip.so")
+
+
+def declared():
+    names = []
+    for fn in os.listdir(os.path.join(ROOT, "include")):
+        if fn.endswith(".h"):
+            text = open(os.path.join(ROOT, "include", fn)).read()
+            text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+            names += re.findall(r"^\s*(?:const\s+)?\w+\s*\*?\s*(\w+)\s*\(", text, flags=re.M)
+    return names
+
+
+def test_header_declares_the_entry_points():
+    names = declared()
+    for n in ("smf_hip_eval", "smf_hip_eval_matrix", "gmf_hip_eval", "smf_hip_version", "smf_hip_target"):
+        assert n in names
+
+
+@pytest.mark.skipif(not os.path.exists(LIB), reason="HIP library not built (python -m matfun_amd.build)")
+def test_library_exports_every_declared_symbol():
+    lib = ctypes.CDLL(LIB)
+    for n in declared():
+        assert hasattr(lib, n), n
+    lib.smf_hip_version.restype = ctypes.c_char_p
+    lib.smf_hip_target.restype = ctypes.c_char_p
+    assert lib.smf_hip_target() == b"gfx950"
+    assert lib.smf_hip_version().startswith(b"0.1.3")
+
+
+@pytest.mark.skipif(not os.path.exists(LIB), reason="HIP library not built (python -m matfun_amd.build)")
+def test_invalid_arguments_rejected_before_any_device_work():
+    """-1 / -2 / n == 0 paths return before touching the device (cuda/tests/test_cuda_api.cu:113-131), so they can be
+    checked on a host without a GPU"""
+    lib = ctypes.CDLL(LIB)
+    args = [ctypes.c_int, ctypes.c_double, ctypes.c_int64] + [ctypes.c_void_p] * 5
+    for f in (lib.smf_hip_eval, lib.smf_hip_eval_matrix, lib.gmf_hip_eval):
+        f.argtypes, f.restype = args, ctypes.c_int
+    dummy = ctypes.c_void_p(0x1000)
+    for f in (lib.smf_hip_eval, lib.smf_hip_eval_matrix, lib.gmf_hip_eval):
+        assert f(9, 0.0, 10, dummy, dummy, None, None, None) == -1          # unknown kind
+        assert f(-1, 0.0, 10, dummy, dummy, None, None, None) == -1
+        assert f(2, 0.0, 10, dummy, dummy, None, None, None) == -1          # pow needs alpha > 0
+        assert f(2, -1.0, 10, dummy, dummy, None, None, None) == -1
+        assert f(2, float("nan"), 10, dummy, dummy, None, None, None) == -1
+        assert f(0, 0.0, 10, dummy, dummy, None, dummy, None) == -1          # H without J
+        assert f(0, 0.0, -1, dummy, dummy, None, None, None) == -1          # n < 0
+        assert f(0, 0.0, 0, dummy, dummy, None, None, None) == 0            # n == 0: nothing to do
+    # grids beyond the limit: -2 without a launch (2^31 blocks of 32 / 64 matrices, and 2^40)
+    assert lib.smf_hip_eval(0, 0.0, (2 ** 31 - 1) * 32 + 1, dummy, dummy, None, None, None) == -2
+    assert lib.smf_hip_eval_matrix(0, 0.0, 2 ** 40, dummy, dummy, None, None, None) == -2
+    assert lib.gmf_hip_eval(0, 0.0, (2 ** 31 - 1) * 64 + 1, dummy, dummy, None, None, None) == -2
