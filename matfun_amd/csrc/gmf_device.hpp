@@ -12,6 +12,12 @@ constexpr int GMF_KMAX = 200;     // series terms at most
 struct GmfCoef {                  // b_j of f(c (1 + u)): exp 1/j!, log (-1)^(j+1)/j, pow binom(al, j); host-built
   double b[GMF_KMAX];
 };
+// (gmf_kernel.hpp:48-53)
+inline void gmf_coefs(int kind, double al, GmfCoef& c) {
+  c.b[0] = 1.0;
+  for (int j = 1; j < GMF_KMAX; j++)
+    c.b[j] = kind == EXP ? c.b[j - 1] / j : (kind == LOG ? ((j & 1) ? 1.0 : -1.0) / j : c.b[j - 1] * (al - j + 1) / j);
+}
 
 // the list of divided differences f[x1^a x2^b x3^c], a >= b >= c, ordered by a, then b, then c (19 entries for
 // a <= 3; the first 3 / 9 / 19 serve orders 0 / 1 / 2).  Each entry adds one node to its parent: the first node
@@ -341,7 +347,7 @@ MF_DEV void gmf_lane(const double* a, double al, const double* __restrict__ bt, 
                          cabs_(mk(x[2].re - cs, x[2].im)));
   const bool triple = group_ok(cs, R3);
   bool psel[3] = {false, false, false};
-  double bestR = __longlong_as_double(0x7ff0000000000000LL);
+  double bestR = dinf();
   constexpr int PI_[3] = {0, 0, 1}, PJ_[3] = {1, 2, 2};
 #pragma unroll
   for (int p = 0; p < 3; p++) {
@@ -526,7 +532,7 @@ MF_DEV void gmf_lane(const double* a, double al, const double* __restrict__ bt, 
                             (i2 == j2 ? b0i * b1i : 0.0);
   const double a01 = (ab[0] || ab[1]) ? mu[0].re - mu[1].re : x[0].re - x[1].re;
   const double N10 = -b0i, N20 = -b0i * a01, N21 = -(b0i + b1i);
-  const double nanf = bad ? __longlong_as_double(0x7ff8000000000000LL) : 1.0;
+  const double nanf = bad ? dnan() : 1.0;
   {
     // F = sum_j Re(T^T d)_j r_j, (T^T v)_0 = v_0 + i (N10 v_1 + N20 v_2), (T^T v)_1 = v_1 + i N21 v_2
     const double Dr0 = D[pos1(0)].re, Dr1 = D[pos1(1)].re, Dr2 = D[pos1(2)].re;
@@ -587,6 +593,55 @@ MF_DEV void gmf_lane(const double* a, double al, const double* __restrict__ bt, 
     }
 #pragma unroll
     for (int k = 0; k < 27; k++) o.D3[k] = Dr[k];
+  }
+}
+
+// ---- assembly of J and H from the staged per-matrix results (r_m[k] at r[9 m + k]); one lane of a 27-lane group
+// J_abcd = sc1 sum_m r_m[a][c] Y_m[d][b], Y_m = sum_n D2_mn r_n  (gmf_kernel.hpp:636-647); lane (b, c, d), a = 0..2
+MF_DEV void gmf_j_lane(const double* r, const double* D2, double sc1, int b, int c, int d, double* out) {
+  double Y[3];
+#pragma unroll
+  for (int mm = 0; mm < 3; mm++)
+    Y[mm] = D2[3 * mm] * r[3 * d + b] + D2[3 * mm + 1] * r[9 + 3 * d + b] + D2[3 * mm + 2] * r[18 + 3 * d + b];
+#pragma unroll
+  for (int aa = 0; aa < 3; aa++)
+    out[aa] = (r[3 * aa + c] * Y[0] + r[9 + 3 * aa + c] * Y[1] + r[18 + 3 * aa + c] * Y[2]) * sc1;
+}
+
+// H_abcdef = sc2 (T6_acdefb + T6_aefcdb), T6_acdefb = sum_mnp D3_mnp r_m[a][c] r_n[d][e] r_p[f][b]
+// (gmf_kernel.hpp:672-691), for lane (d, e, f) and (a, b, c) = 0..26: with Z[mn][fb] = sum_p D3_mnp r_p[f][b] and
+// Yh[m][(de)(fb)] = sum_n r_n[d][e] Z[mn][fb], H = sc2 (sum_m r_m[a][c] X1[m][b] + sum_m r_m[a][e] X2[m][c][b]),
+// X1[m][b] = Yh[m][(de)(fb)], X2[m][c][b] = Yh[m][(fc)(db)]; out[9a + 3b + c]
+MF_DEV void gmf_h_lane(const double* r, const double* D3, double sc2, int d, int e, int f, double* out) {
+  double X1[3][3], X2[3][3][3];
+#pragma unroll
+  for (int mm = 0; mm < 3; mm++) {
+    double Z1[3][3], Z2[3][3];   // [n][b]: Z[mn][fb], Z[mn][db]
+#pragma unroll
+    for (int nn = 0; nn < 3; nn++)
+#pragma unroll
+      for (int b = 0; b < 3; b++) {
+        const double* D = D3 + 9 * mm + 3 * nn;
+        Z1[nn][b] = D[0] * r[3 * f + b] + D[1] * r[9 + 3 * f + b] + D[2] * r[18 + 3 * f + b];
+        Z2[nn][b] = D[0] * r[3 * d + b] + D[1] * r[9 + 3 * d + b] + D[2] * r[18 + 3 * d + b];
+      }
+#pragma unroll
+    for (int b = 0; b < 3; b++) {
+      X1[mm][b] = r[3 * d + e] * Z1[0][b] + r[9 + 3 * d + e] * Z1[1][b] + r[18 + 3 * d + e] * Z1[2][b];
+#pragma unroll
+      for (int c = 0; c < 3; c++)
+        X2[mm][c][b] = r[3 * f + c] * Z2[0][b] + r[9 + 3 * f + c] * Z2[1][b] + r[18 + 3 * f + c] * Z2[2][b];
+    }
+  }
+#pragma unroll
+  for (int aa = 0; aa < 3; aa++) {
+    const double re0 = r[3 * aa + e], re1 = r[9 + 3 * aa + e], re2 = r[18 + 3 * aa + e];
+#pragma unroll
+    for (int b = 0; b < 3; b++)
+#pragma unroll
+      for (int c = 0; c < 3; c++)
+        out[9 * aa + 3 * b + c] = (r[3 * aa + c] * X1[0][b] + r[9 + 3 * aa + c] * X1[1][b] + r[18 + 3 * aa + c] * X1[2][b] +
+                                   re0 * X2[0][c][b] + re1 * X2[1][c][b] + re2 * X2[2][c][b]) * sc2;
   }
 }
 

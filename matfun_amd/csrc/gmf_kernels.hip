@@ -66,69 +66,26 @@ __global__ void __launch_bounds__(GWAVE) gmf_kernel(int64_t n, const double* __r
   constexpr int G = 27, MPP = GWAVE / G;
   const int g = lane / G, L = lane - g * G;
   if (g < MPP) {
-    const int b = L / 9, c = (L / 3) % 3, d = L % 3;
     for (int j = g; j < nm; j += MPP) {
       const double* s = tile + j * T::STRIDE;
-      const double* r = s + T::R;
-      const double* D2 = s + T::D2;
-      double Y[3];
-#pragma unroll
-      for (int mm = 0; mm < 3; mm++)
-        Y[mm] = D2[3 * mm] * r[3 * d + b] + D2[3 * mm + 1] * r[9 + 3 * d + b] + D2[3 * mm + 2] * r[18 + 3 * d + b];
-      const double sc1 = s[T::SC1];
+      double out[3];
+      gmf_j_lane(s + T::R, s + T::D2, s[T::SC1], L / 9, (L / 3) % 3, L % 3, out);
       double* o = J + (base + j) * 81 + L;
 #pragma unroll
-      for (int aa = 0; aa < 3; aa++)
-        __builtin_nontemporal_store((r[3 * aa + c] * Y[0] + r[9 + 3 * aa + c] * Y[1] + r[18 + 3 * aa + c] * Y[2]) * sc1,
-                                    o + 27 * aa);
+      for (int aa = 0; aa < 3; aa++) __builtin_nontemporal_store(out[aa], o + 27 * aa);
     }
   }
   if (ORDER < 2) return;
 
-  // ---- H: lane (d, e, f) = 9d + 3e + f writes H_abcdef for (a, b, c) = 0..26 (positions 27 (9a + 3b + c) + lane):
-  //      H_abcdef = sc2 (sum_m r_m[a][c] X1[m][b] + sum_m r_m[a][e] X2[m][c][b]) with
-  //      X1[m][b] = Yh[m][(de)(fb)] and X2[m][c][b] = Yh[m][(fc)(db)], Yh[m][(de)(fb)] = sum_n r_n[de] Z[mn][fb],
-  //      Z[mn][fb] = sum_p D3[mnp] r_p[fb]   (the contractions of gmf_kernel.hpp:672-691)
+  // ---- H: lane (d, e, f) = 9d + 3e + f writes H_abcdef for (a, b, c) = 0..26 (positions 27 (9a + 3b + c) + lane)
   if (g < MPP) {
-    const int d = L / 9, e = (L / 3) % 3, f = L % 3;
     for (int j = g; j < nm; j += MPP) {
       const double* s = tile + j * T::STRIDE;
-      const double* r = s + T::R;
-      const double* D3 = s + T::D3;
-      double X1[3][3], X2[3][3][3];
-#pragma unroll
-      for (int mm = 0; mm < 3; mm++) {
-        double Z1[3][3], Z2[3][3];   // [n][b]: Z[mn][fb] and Z[mn][db]
-#pragma unroll
-        for (int nn = 0; nn < 3; nn++)
-#pragma unroll
-          for (int b = 0; b < 3; b++) {
-            const double* D = D3 + 9 * mm + 3 * nn;
-            Z1[nn][b] = D[0] * r[3 * f + b] + D[1] * r[9 + 3 * f + b] + D[2] * r[18 + 3 * f + b];
-            Z2[nn][b] = D[0] * r[3 * d + b] + D[1] * r[9 + 3 * d + b] + D[2] * r[18 + 3 * d + b];
-          }
-#pragma unroll
-        for (int b = 0; b < 3; b++) {
-          X1[mm][b] = r[3 * d + e] * Z1[0][b] + r[9 + 3 * d + e] * Z1[1][b] + r[18 + 3 * d + e] * Z1[2][b];
-#pragma unroll
-          for (int c = 0; c < 3; c++)
-            X2[mm][c][b] = r[3 * f + c] * Z2[0][b] + r[9 + 3 * f + c] * Z2[1][b] + r[18 + 3 * f + c] * Z2[2][b];
-        }
-      }
-      const double sc2 = s[T::SC2];
+      double out[27];
+      gmf_h_lane(s + T::R, s + T::D3, s[T::SC2], L / 9, (L / 3) % 3, L % 3, out);
       double* o = H + (base + j) * 729 + L;
 #pragma unroll
-      for (int aa = 0; aa < 3; aa++) {
-        const double re0 = r[3 * aa + e], re1 = r[9 + 3 * aa + e], re2 = r[18 + 3 * aa + e];
-#pragma unroll
-        for (int b = 0; b < 3; b++)
-#pragma unroll
-          for (int c = 0; c < 3; c++) {
-            const double h = r[3 * aa + c] * X1[0][b] + r[9 + 3 * aa + c] * X1[1][b] + r[18 + 3 * aa + c] * X1[2][b] +
-                             re0 * X2[0][c][b] + re1 * X2[1][c][b] + re2 * X2[2][c][b];
-            __builtin_nontemporal_store(h * sc2, o + 27 * (9 * aa + 3 * b + c));
-          }
-      }
+      for (int k = 0; k < 27; k++) __builtin_nontemporal_store(out[k], o + 27 * k);
     }
   }
 }

@@ -15,27 +15,6 @@
 
 namespace {
 
-// Taylor coefficients of the second divided differences, f^(k+2)(m) / (k+2)! without the m-dependent factor:
-// exp 1/(k+2)!, log (-1)^(k+1)/(k+2), powers binom(al, k+2)  (smf_kernel.hpp:135-146)
-void smf_coefs(int kind, double al, mf::SmfCoef& c) {
-  if (kind == mf::EXP) {
-    double f = 0.5;
-    for (int k = 0; k < mf::SMF_NTERMS; k++) { c.c[k] = f; f /= (k + 3.0); }
-  } else if (kind == mf::LOG) {
-    for (int k = 0; k < mf::SMF_NTERMS; k++) c.c[k] = ((k & 1) ? 1.0 : -1.0) / (k + 2.0);
-  } else {
-    double b = al * (al - 1.0) / 2.0;
-    for (int k = 0; k < mf::SMF_NTERMS; k++) { c.c[k] = b; b *= (al - k - 2.0) / (k + 3.0); }
-  }
-}
-
-// b_j of f(c (1 + u)): exp 1/j!, log (-1)^(j+1)/j, pow binom(al, j)  (gmf_kernel.hpp:48-53)
-void gmf_coefs(int kind, double al, mf::GmfCoef& c) {
-  c.b[0] = 1.0;
-  for (int j = 1; j < mf::GMF_KMAX; j++)
-    c.b[j] = kind == mf::EXP ? c.b[j - 1] / j : (kind == mf::LOG ? ((j & 1) ? 1.0 : -1.0) / j : c.b[j - 1] * (al - j + 1) / j);
-}
-
 bool valid(int kind, double alpha, int64_t n, const double* J, const double* H) {
   return !(kind < mf::EXP || kind > mf::SQRT || n < 0 || (H && !J) || (kind == mf::POW && !(alpha > 0)));
 }
@@ -56,7 +35,7 @@ int smf_call(int kind, double alpha, int64_t n, const double* x, bool mat, doubl
   if (n == 0) return 0;
   if (n > SMF_MAX_N) return -2;
   mf::SmfCoef c2;
-  smf_coefs(kind, alpha, c2);
+  mf::smf_coefs(kind, alpha, c2);
   const int nin = mat ? 9 : 6, nf = mat ? 9 : 6;
   for (int64_t lo = 0; lo < n; lo += mf::MAX_PER_LAUNCH) {
     const int64_t m = n - lo < mf::MAX_PER_LAUNCH ? n - lo : mf::MAX_PER_LAUNCH;
@@ -87,7 +66,7 @@ extern "C" int gmf_hip_eval(int kind, double alpha, int64_t n, const double* A, 
   if (n == 0) return 0;
   if (n > GMF_MAX_N) return -2;
   mf::GmfCoef bt;
-  gmf_coefs(kind, alpha, bt);
+  mf::gmf_coefs(kind, alpha, bt);
   for (int64_t lo = 0; lo < n; lo += mf::MAX_PER_LAUNCH) {
     const int64_t m = n - lo < mf::MAX_PER_LAUNCH ? n - lo : mf::MAX_PER_LAUNCH;
     mf::gmf_launch((hipStream_t)stream, kind, m, A + 9 * lo, F + 9 * lo, J ? J + 81 * lo : nullptr,

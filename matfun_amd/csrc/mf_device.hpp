@@ -3,7 +3,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
-#define MF_DEV __device__ __forceinline__
+// (host and device: the per-lane math is also compiled for the CPU to check it there, tests/lanecheck)
+#define MF_DEV __host__ __device__ __forceinline__
 
 namespace mf {
 
@@ -11,13 +12,16 @@ enum Kind { EXP = 0, LOG = 1, POW = 2, SQRT = 3 };
 
 // largest power of two <= x for a positive normal x: the exponent bits of x alone (exact)
 MF_DEV double pow2_floor(double x) {
-  return __longlong_as_double(__double_as_longlong(x) & 0x7ff0000000000000LL);
+  return __builtin_bit_cast(double, __builtin_bit_cast(long long, x) & 0x7ff0000000000000LL);
 }
+MF_DEV bool sign_bit(double x) { return __builtin_bit_cast(long long, x) < 0; }
+MF_DEV double dinf() { return __builtin_bit_cast(double, 0x7ff0000000000000LL); }
+MF_DEV double dnan() { return __builtin_bit_cast(double, 0x7ff8000000000000LL); }
 // a power of two within a factor 2 of x; 1 outside (1e-300, 1e300) and for NaN
 MF_DEV double pow2_near(double x) { return (x > 1e-300 && x < 1e300) ? pow2_floor(x) : 1.0; }
 
 // 0^e: inf (e < 0), 1 (e = 0), 0 (e > 0)
-MF_DEV double pow_of_zero(double e) { return e < 0 ? __longlong_as_double(0x7ff0000000000000LL) : (e == 0 ? 1.0 : 0.0); }
+MF_DEV double pow_of_zero(double e) { return e < 0 ? dinf() : (e == 0 ? 1.0 : 0.0); }
 
 // x^(al - k), x > 0, with the exponent al - k = bh + bl split exactly (TwoSum): the rounding of al - k alone would
 // cost |log x| times its error; x^(bh + bl) = x^bh (1 + bl log x)

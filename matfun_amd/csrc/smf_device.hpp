@@ -12,6 +12,18 @@ constexpr int SMF_NTERMS = 30;   // terms of the second-divided-difference Taylo
 struct SmfCoef {                  // f^(k+2)(m) / (k+2)! without the m-dependent factor, built on the host per call
   double c[SMF_NTERMS];
 };
+// exp 1/(k+2)!, log (-1)^(k+1)/(k+2), powers binom(al, k+2)  (smf_kernel.hpp:135-146)
+inline void smf_coefs(int kind, double al, SmfCoef& c) {
+  if (kind == EXP) {
+    double f = 0.5;
+    for (int k = 0; k < SMF_NTERMS; k++) { c.c[k] = f; f /= (k + 3.0); }
+  } else if (kind == LOG) {
+    for (int k = 0; k < SMF_NTERMS; k++) c.c[k] = ((k & 1) ? 1.0 : -1.0) / (k + 2.0);
+  } else {
+    double b = al * (al - 1.0) / 2.0;
+    for (int k = 0; k < SMF_NTERMS; k++) { c.c[k] = b; b *= (al - k - 2.0) / (k + 3.0); }
+  }
+}
 
 // Voigt index pairs: k -> (row, col) of the symmetric 3x3 matrix
 MF_DEV constexpr int vrow(int k) { return k < 3 ? k : (k == 5 ? 1 : 0); }
@@ -195,7 +207,7 @@ struct SmfLane {
     const double jrs = jz ? 1.0 : jr;
     const double big = sqrt((jrs + fabs(jx)) / (2.0 * jrs));
     const double sml = jy / (2.0 * jrs * big);
-    const double sgn = ((jy > 0) || ((jy == 0) && !signbit(jy))) ? 1.0 : -1.0;
+    const double sgn = ((jy > 0) || ((jy == 0) && !sign_bit(jy))) ? 1.0 : -1.0;
     const double cs = jz ? 1.0 : (jx >= 0 ? big : sgn * sml);
     const double sn = jz ? 0.0 : (jx >= 0 ? sml : sgn * big);
     lam[0] = lv * sig;
@@ -321,7 +333,7 @@ MF_DEV void matrix_to_voigt(const double* a, double* v) {
   for (int k = 0; k < 9; k++) mx = fabs(a[k]) > mx ? fabs(a[k]) : mx;
   const double tol = 1e-8 * mx;
   const bool sym = !(fabs(a[1] - a[3]) > tol) && !(fabs(a[2] - a[6]) > tol) && !(fabs(a[5] - a[7]) > tol);
-  const double nan = __longlong_as_double(0x7ff8000000000000LL);
+  const double nan = dnan();
   v[0] = sym ? a[0] : nan;
   v[1] = sym ? a[4] : nan;
   v[2] = sym ? a[8] : nan;
