@@ -1,17 +1,21 @@
 """Benchmark of the batched matrix-function evaluation on one MI355X.
 
-Workload (default): symmetric 3x3 matrices, matrix layout, order 2 (F 3x3, J 3^4, H 3^6), FP64, n = 2^22 per function
-for exp, log, sqrt and pow(0.3); inputs A = Q diag(lam) Q^T, Q Haar-random, lam log-uniform in [1e-2, 1e2], with
-15% exactly double, 5% triple and 10% 1e-8-gap eigenvalues (seed 0).  One step = the four functions over the n
+Headline workload: symmetric 3x3 matrices, matrix layout, order 2 (F 3x3, J 3^4, H 3^6), FP64, n = 2^22 per
+function for exp, log, sqrt and pow(0.3); inputs A = Q diag(lam) Q^T, Q Haar-random, lam log-uniform in [1e-2, 1e2],
+with 15% exactly double, 5% triple and 10% 1e-8-gap eigenvalues (seed 0).  One step = the four functions over the n
 matrices (one smf_hip_eval_matrix call each), inputs resident in HBM, outputs overwritten each step.
 
-    python bench.py [--steps K] [--warmup W] [--n N] [--gpus 1] [--no-cpu] [--gmf]
+Also measured (same JSON line, key "legs"): the mixed form of the same workload (FP32 outputs; log, sqrt, pow(0.3):
+exp of eigenvalues up to 1e2 is beyond the FP32 range) with its error against the FP64 form, and general matrices
+(gmf_hip_eval, n = 2^20 per function, similarity / complex-pair / shear / 4I+N mix, seed 0) in FP64 and mixed form.
 
-Prints one JSON line: value = matrices/s of the whole step (4 n matrices per step); `roofline` for the dominant
-kernel (smf order-2 matrix-layout kernel): algorithmic bytes per launch (A in + F, J, H out: 6624 B per matrix) over
-its average launch time measured with HIP events on its stream; `cpu_baseline` the reference C library (built from
-the reference sources into oracle/_ref, SIMD + OpenMP) or, without it, this repository's scalar C oracle, on a bounded
-sample of the same workload on the host cores.
+    python bench.py [--steps K] [--warmup W] [--n N] [--gmf-n N] [--gpus 1] [--no-cpu] [--only-headline]
+
+`value` = matrices/s of the headline step (4 n matrices per step).  `roofline`: the dominant kernel (the smf order-2
+matrix-layout FP64 kernel): algorithmic bytes per launch (A in + F, J, H out: 6624 B per matrix) over its average
+launch time, measured with HIP events on the launch stream.  `cpu_baseline`: the reference C library (built from the
+reference sources into oracle/_ref; SIMD + OpenMP) or, without it, this repository's scalar C oracle, on a bounded
+sample of the headline inputs on the host cores.
 """
 import argparse
 import json
@@ -26,8 +30,15 @@ sys.path.insert(0, ROOT)
 
 METRIC = "matrices/s at order 2 (FP64 vs mixed, ≤1e-5 rel err), % HBM roofline"
 HBM_PEAK_GBS = 8000.0
-BYTES_O2_MATRIX = 72 + 72 + 648 + 5832          # A in, F, J, H out (FP64, matrix layout)
 FUNCS = [("exp", 0.0), ("log", 0.0), ("sqrt", 0.0), ("pow", 0.3)]
+
+
+def fname(k, al):
+    return k if k != "pow" else f"pow{al:g}"
+
+
+def bytes_per_matrix(order, out_bytes):
+    return 72 + out_bytes * (9 + (81 if order >= 1 else 0) + (729 if order >= 2 else 0))
 
 
 def log(*a):
@@ -41,7 +52,7 @@ def cpu_baseline(A, seconds=10.0, chunk=1 << 17):
     chk = Checker("ref" if kind == "reference" else "oracle")
     threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
     chk.set_threads(threads)
-    done, t0, lo = 0, time.perf_counter(), 0
+    done, lo = 0, 0
     chk.smf_matrix("log", 0.0, A[:256], 2)                         # warm-up (libmvec binding, thread pool)
     t0 = time.perf_counter()
     while time.perf_counter() - t0 < seconds and lo < len(A):
@@ -53,8 +64,8 @@ def cpu_baseline(A, seconds=10.0, chunk=1 << 17):
         lo += chunk
     dt = time.perf_counter() - t0
     return {"value": done / dt, "unit": "matrices/s", "cores": threads, "kind": kind,
-            "sample": f"{done} matrices ({done // 4} per function, first rows of the same inputs), {chk.name()}, "
-                      f"{dt:.1f} s wall"}
+            "sample": f"{done} matrices ({done // 4} per function, first rows of the headline inputs), {chk.name()}, "
+                      f"order 2 matrix layout, {dt:.1f} s wall"}
 
 
 def main():
@@ -63,12 +74,14 @@ def main():
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--n", type=int, default=1 << 22)
+    ap.add_argument("--gmf-n", type=int, default=1 << 20)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--only-headline", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU only; --gpus must be 1 (the batch shards trivially "
-                                   "by matrix ranges, see DESIGN.md)"}))
+                                   "by contiguous matrix ranges, one process per GPU; see DESIGN.md)"}))
         sys.exit(2)
 
     import torch
@@ -78,46 +91,91 @@ def main():
     dev = torch.device("cuda:0")
     torch.cuda.set_device(dev)
     lib = matfun_amd.library()
-    n = args.n
-    log(f"bench: generating {n} symmetric inputs (seed 0)")
+    n, ng = args.n, args.gmf_n
+    log(f"bench: generating {n} symmetric and {ng} general inputs (seed 0)")
     A = W.spd(n, seed=0)
     dA = torch.from_numpy(A).to(dev)
-    F = torch.empty((n, 9), dtype=torch.float64, device=dev)
-    J = torch.empty((n, 81), dtype=torch.float64, device=dev)
-    H = torch.empty((n, 729), dtype=torch.float64, device=dev)
+    dG = torch.from_numpy(W.general(ng, seed=0)).to(dev)
+    # one set of output buffers, FP64-sized; the mixed form writes FP32 into the same memory
+    Fb = torch.empty(n * 9, dtype=torch.float64, device=dev)
+    Jb = torch.empty(n * 81, dtype=torch.float64, device=dev)
+    Hb = torch.empty(n * 729, dtype=torch.float64, device=dev)
     stream = torch.cuda.current_stream()
     sh = stream.cuda_stream
-    ev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in FUNCS]
 
-    def step(record):
-        for i, (k, al) in enumerate(FUNCS):
-            if record:
-                ev[i][0].record(stream)
-            rc = lib.smf_hip_eval_matrix(matfun_amd.KINDS[k], al, n, dA.data_ptr(), F.data_ptr(), J.data_ptr(),
-                                         H.data_ptr(), sh)
-            if record:
-                ev[i][1].record(stream)
-            assert rc == 0, rc
+    def leg(name, fn, X, m, out_bytes, steps, warmup, funcs=FUNCS):
+        ev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in funcs]
+        ptrs = (Fb.data_ptr(), Jb.data_ptr(), Hb.data_ptr())
 
-    for _ in range(args.warmup):
-        step(False)
-    torch.cuda.synchronize()
-    kern_ms = []
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
-        step(True)
+        def step(record):
+            for i, (k, al) in enumerate(funcs):
+                if record:
+                    ev[i][0].record(stream)
+                rc = fn(matfun_amd.KINDS[k], al, m, X.data_ptr(), *ptrs, sh)
+                if record:
+                    ev[i][1].record(stream)
+                assert rc == 0, (name, rc)
+
+        for _ in range(warmup):
+            step(False)
         torch.cuda.synchronize()
-        kern_ms += [a.elapsed_time(b) for a, b in ev]
-    torch.cuda.synchronize()
-    wall = time.perf_counter() - t0
-    ms_per_step = wall * 1e3 / args.steps
-    value = 4 * n / (wall / args.steps)
-    avg_kernel_ms = float(np.mean(kern_ms))
-    achieved = n * BYTES_O2_MATRIX / (avg_kernel_ms * 1e-3) / 1e9
-    per_func = {k if k != "pow" else "pow0.3": n / (float(np.mean(kern_ms[i::4])) * 1e-3) for i, (k, _) in enumerate(FUNCS)}
-    log(f"bench: {ms_per_step:.2f} ms/step, kernel avg {avg_kernel_ms:.3f} ms, {achieved:.0f} GB/s")
+        kern = []
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            step(True)
+            torch.cuda.synchronize()
+            kern += [a.elapsed_time(b) for a, b in ev]
+        wall = (time.perf_counter() - t0) / steps
+        kms = float(np.mean(kern))
+        bpm = bytes_per_matrix(2, out_bytes)
+        gbs = m * bpm / (kms * 1e-3) / 1e9
+        nf = len(funcs)
+        per = {fname(k, al): m / (float(np.mean(kern[i::nf])) * 1e-3) for i, (k, al) in enumerate(funcs)}
+        log(f"bench: {name}: {wall * 1e3:.2f} ms/step, kernel avg {kms:.3f} ms, {gbs:.0f} GB/s "
+            f"({gbs / HBM_PEAK_GBS:.1%} of HBM peak), {nf * m / wall / 1e6:.1f} M matrices/s")
+        return {"functions": [fname(k, al) for k, al in funcs], "matrices_per_s": nf * m / wall,
+                "ms_per_step": wall * 1e3, "n_per_function": m, "bytes_per_matrix": bpm, "avg_kernel_ms": kms,
+                "achieved_GBs": gbs, "frac": gbs / HBM_PEAK_GBS, "per_function_matrices_per_s": per}
 
-    # device copy peak of the same byte count order (read + write), for context
+    def mixed_error(fn64, fnmx, X, funcs, m=16384):
+        """max over m matrices of the per-matrix normwise relative error of the mixed form against the FP64 form
+        (itself checked against the CPU oracle by the test suite), per function and output"""
+        res = {}
+        for k, al in funcs:
+            outs = []
+            for fn, dt in ((fn64, torch.float64), (fnmx, torch.float32)):
+                o = [torch.empty((m, s), dtype=dt, device=dev) for s in (9, 81, 729)]
+                assert fn(matfun_amd.KINDS[k], al, m, X.data_ptr(), *[t.data_ptr() for t in o], sh) == 0
+                outs.append(o)
+            torch.cuda.synchronize()
+            e = {}
+            for nm, a, b in zip("FJH", outs[0], outs[1]):
+                err = torch.linalg.norm(b.double() - a, dim=1) / torch.linalg.norm(a, dim=1)
+                e[nm] = float(err.max())
+            res[fname(k, al)] = e
+        return res
+
+    funcs_mx = FUNCS[1:]          # (exp of eigenvalues up to 1e2 is beyond the FP32 range of the mixed form)
+    head = leg("smf fp64", lib.smf_hip_eval_matrix, dA, n, 8, args.steps, args.warmup)
+    legs = {"smf_fp64": head}
+    if not args.only_headline:
+        legs["smf_mixed"] = leg("smf mixed", lib.smf_hip_eval_matrix_mixed, dA, n, 4, args.steps, args.warmup,
+                                funcs_mx)
+        legs["gmf_fp64"] = leg("gmf fp64", lib.gmf_hip_eval, dG, ng, 8, args.steps, args.warmup)
+        legs["gmf_mixed"] = leg("gmf mixed", lib.gmf_hip_eval_mixed, dG, ng, 4, args.steps, args.warmup)
+        legs["smf_mixed"]["max_err_vs_fp64"] = mixed_error(lib.smf_hip_eval_matrix, lib.smf_hip_eval_matrix_mixed,
+                                                           dA, funcs_mx)
+        legs["gmf_mixed"]["max_err_vs_fp64"] = mixed_error(lib.gmf_hip_eval, lib.gmf_hip_eval_mixed, dG, FUNCS)
+        # FP64 vs mixed on the same functions: summed time per step of the mixed legs' functions
+        t64_smf = sum(n / head["per_function_matrices_per_s"][fname(k, al)] for k, al in funcs_mx)
+        tmx_smf = len(funcs_mx) * n / legs["smf_mixed"]["matrices_per_s"]
+        t64 = t64_smf + 4 * ng / legs["gmf_fp64"]["matrices_per_s"]
+        tmx = tmx_smf + 4 * ng / legs["gmf_mixed"]["matrices_per_s"]
+        legs["smf_mixed"]["speedup_vs_fp64"] = t64_smf / tmx_smf
+        legs["gmf_mixed"]["speedup_vs_fp64"] = legs["gmf_mixed"]["matrices_per_s"] / legs["gmf_fp64"]["matrices_per_s"]
+        legs["mixed_vs_fp64_smf_plus_gmf"] = t64 / tmx
+
+    # device copy bandwidth (read + write of 2 GiB), for context next to the 8 TB/s spec
     x = torch.empty(1 << 28, dtype=torch.float64, device=dev)
     y = torch.empty_like(x)
     y.copy_(x)
@@ -131,17 +189,17 @@ def main():
     copy_gbs = 5 * 2 * x.numel() * 8 / (c0.elapsed_time(c1) * 1e-3) / 1e9
     del x, y
 
-    out = {"metric": METRIC, "value": value, "unit": "matrices/s", "n_gpus": 1, "steps": args.steps,
-           "warmup": args.warmup, "ms_per_step": ms_per_step, "higher_is_better": True, "scaling": "weak",
+    out = {"metric": METRIC, "value": head["matrices_per_s"], "unit": "matrices/s", "n_gpus": 1, "steps": args.steps,
+           "warmup": args.warmup, "ms_per_step": head["ms_per_step"], "higher_is_better": True, "scaling": "weak",
            "vs_baseline": None, "dtype": "f64", "data": "synthetic",
            "config": {"workload": "smf exp/log/sqrt/pow(0.3), matrix layout, order 2 (F 3x3, J 3^4, H 3^6), FP64, "
                                   "n=2^22 per function, SPD Q diag(lam) Q^T with exact/near repeated eigenvalues",
-                      "n_per_function": n, "functions": [f"{k}{al if k == 'pow' else ''}" for k, al in FUNCS],
-                      "order": 2, "layout": "matrix", "bytes_per_matrix": BYTES_O2_MATRIX,
-                      "per_function_matrices_per_s": per_func},
-           "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                        "frac": achieved / HBM_PEAK_GBS, "traffic": None, "kernel": "mf::smf_kernel<KIND,true,2>",
-                        "avg_kernel_ms": avg_kernel_ms, "measured_copy_GBs": copy_gbs}}
+                      "n_per_function": n, "functions": ["exp", "log", "sqrt", "pow0.3"], "order": 2,
+                      "layout": "matrix", "bytes_per_matrix": head["bytes_per_matrix"]},
+           "roofline": {"bound": "hbm", "achieved": head["achieved_GBs"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                        "frac": head["frac"], "traffic": None, "kernel": "mf::smf_kernel<KIND,true,2,double>",
+                        "avg_kernel_ms": head["avg_kernel_ms"], "measured_copy_GBs": copy_gbs},
+           "legs": legs}
     if not args.no_cpu:
         log("bench: CPU baseline")
         out["cpu_baseline"] = cpu_baseline(A, seconds=args.cpu_seconds)

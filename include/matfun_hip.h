@@ -39,6 +39,16 @@ int smf_hip_eval_matrix(int kind, double alpha, int64_t n, const double *A, doub
  * (matfun_cuda.h:21, matfun_cuda.cu:194-197). */
 int gmf_hip_eval(int kind, double alpha, int64_t n, const double *A, double *F, double *J, double *H, void *stream);
 
+/* ---- mixed-precision form (no counterpart in the reference API): the same inputs (FP64), arguments, layouts and
+ * return codes, outputs in FP32.  The eigen-decomposition, f and the divided differences (with their gap tests and
+ * Taylor branches) are FP64; the derivative assembly and all stored values are FP32, which halves the output bytes
+ * that bound order 1 and 2.  Normwise relative error against the FP64 form about 1e-7 (F) / 1e-6 (J, H) on
+ * well-conditioned input; results must lie in the FP32 range. */
+int smf_hip_eval_mixed(int kind, double alpha, int64_t n, const double *v, float *F, float *J, float *H, void *stream);
+int smf_hip_eval_matrix_mixed(int kind, double alpha, int64_t n, const double *A, float *F, float *J, float *H,
+                              void *stream);
+int gmf_hip_eval_mixed(int kind, double alpha, int64_t n, const double *A, float *F, float *J, float *H, void *stream);
+
 /* library version ("0.1.3"-compatible API level plus this build); replaces smf_version (c/include/matfun/matfun.h:53) */
 const char *smf_hip_version(void);
 

@@ -36,7 +36,8 @@ def library():
             raise RuntimeError(f"matfun_amd: HIP library {path} not built (python -m matfun_amd.build)")
         lib = ctypes.CDLL(path)
         args = [ctypes.c_int, ctypes.c_double, ctypes.c_int64] + [ctypes.c_void_p] * 5
-        for fn in (lib.smf_hip_eval, lib.smf_hip_eval_matrix, lib.gmf_hip_eval):
+        for fn in (lib.smf_hip_eval, lib.smf_hip_eval_matrix, lib.gmf_hip_eval, lib.smf_hip_eval_mixed,
+                   lib.smf_hip_eval_matrix_mixed, lib.gmf_hip_eval_mixed):
             fn.argtypes, fn.restype = args, ctypes.c_int
         lib.smf_hip_version.restype = ctypes.c_char_p
         lib.smf_hip_target.restype = ctypes.c_char_p
@@ -64,14 +65,14 @@ def _prepare(kind, alpha, order):
         raise ValueError("order must be 0, 1 or 2")
 
 
-def _run(fn, kind, alpha, A, batch_shape, shapes, order):
+def _run(fn, kind, alpha, A, batch_shape, shapes, order, out_dtype):
     import torch
     to_numpy = not isinstance(A, torch.Tensor) or not A.is_cuda
     X = torch.as_tensor(A, dtype=torch.float64)
     if not X.is_cuda:
         X = X.to("cuda")
     X = X.contiguous()
-    out = [torch.empty(batch_shape + s, dtype=torch.float64, device=X.device) for s in shapes[:order + 1]]
+    out = [torch.empty(batch_shape + s, dtype=out_dtype, device=X.device) for s in shapes[:order + 1]]
     ptrs = [o.data_ptr() for o in out] + [None] * (2 - order)
     n = math.prod(batch_shape)
     with torch.cuda.device(X.device):
@@ -83,25 +84,40 @@ def _run(fn, kind, alpha, A, batch_shape, shapes, order):
     return tuple(out) if order else out[0]
 
 
-def smf(kind, A, alpha=None, order=0):
+def _precision(precision):
+    import torch
+    if precision not in ("fp64", "mixed"):
+        raise ValueError("precision must be 'fp64' or 'mixed'")
+    return precision == "mixed", (torch.float32 if precision == "mixed" else torch.float64)
+
+
+def smf(kind, A, alpha=None, order=0, precision="fp64"):
     """f(A) and its derivatives for symmetric 3x3 matrices A (..., 3, 3) or Voigt vectors (..., 6) =
     (A11, A22, A33, A12, A13, A23); results in the same form: F (..., 3, 3), J (..., 3, 3, 3, 3) = dF_ab/dA_cd,
-    H (..., 3^6) resp. F (..., 6), J (..., 6, 6), H (..., 6, 6, 6).  order 0: F, 1: (F, J), 2: (F, J, H)."""
+    H (..., 3^6) resp. F (..., 6), J (..., 6, 6), H (..., 6, 6, 6).  order 0: F, 1: (F, J), 2: (F, J, H).
+    precision "mixed": FP32 results (FP64 eigen-decomposition and divided differences, FP32 assembly)."""
     _prepare(kind, alpha, order)
+    mixed, dt = _precision(precision)
+    lib = library()
     shape = tuple(A.shape)
     if shape[-2:] == (3, 3):
-        return _run(library().smf_hip_eval_matrix, kind, alpha, A, shape[:-2],
-                    ((3, 3), (3, 3, 3, 3), (3, 3, 3, 3, 3, 3)), order)
+        return _run(lib.smf_hip_eval_matrix_mixed if mixed else lib.smf_hip_eval_matrix, kind, alpha, A, shape[:-2],
+                    ((3, 3), (3, 3, 3, 3), (3, 3, 3, 3, 3, 3)), order, dt)
     if shape[-1:] == (6,):
-        return _run(library().smf_hip_eval, kind, alpha, A, shape[:-1], ((6,), (6, 6), (6, 6, 6)), order)
+        return _run(lib.smf_hip_eval_mixed if mixed else lib.smf_hip_eval, kind, alpha, A, shape[:-1],
+                    ((6,), (6, 6), (6, 6, 6)), order, dt)
     raise ValueError(f"expected matrices (..., 3, 3) or Voigt vectors (..., 6), got shape {shape}")
 
 
-def gmf(kind, A, alpha=None, order=0):
+def gmf(kind, A, alpha=None, order=0, precision="fp64"):
     """f(A) and its derivatives for general real 3x3 matrices A (..., 3, 3), principal branch; NaN results for a
-    matrix with a real eigenvalue <= 0 (log, sqrt, pow).  order 0: F, 1: (F, J), 2: (F, J, H)."""
+    matrix with a real eigenvalue <= 0 (log, sqrt, pow).  order 0: F, 1: (F, J), 2: (F, J, H).
+    precision "mixed": FP32 results (FP64 per-matrix spectral part, FP32 assembly of J and H)."""
     _prepare(kind, alpha, order)
+    mixed, dt = _precision(precision)
+    lib = library()
     shape = tuple(A.shape)
     if shape[-2:] != (3, 3):
         raise ValueError(f"expected matrices (..., 3, 3), got shape {shape}")
-    return _run(library().gmf_hip_eval, kind, alpha, A, shape[:-2], ((3, 3), (3, 3, 3, 3), (3, 3, 3, 3, 3, 3)), order)
+    return _run(lib.gmf_hip_eval_mixed if mixed else lib.gmf_hip_eval, kind, alpha, A, shape[:-2],
+                ((3, 3), (3, 3, 3, 3), (3, 3, 3, 3, 3, 3)), order, dt)

@@ -598,8 +598,9 @@ MF_DEV void gmf_lane(const double* a, double al, const double* __restrict__ bt, 
 
 // ---- assembly of J and H from the staged per-matrix results (r_m[k] at r[9 m + k]); one lane of a 27-lane group
 // J_abcd = sc1 sum_m r_m[a][c] Y_m[d][b], Y_m = sum_n D2_mn r_n  (gmf_kernel.hpp:636-647); lane (b, c, d), a = 0..2
-MF_DEV void gmf_j_lane(const double* r, const double* D2, double sc1, int b, int c, int d, double* out) {
-  double Y[3];
+template <class T>
+MF_DEV void gmf_j_lane(const T* r, const T* D2, T sc1, int b, int c, int d, T* out) {
+  T Y[3];
 #pragma unroll
   for (int mm = 0; mm < 3; mm++)
     Y[mm] = D2[3 * mm] * r[3 * d + b] + D2[3 * mm + 1] * r[9 + 3 * d + b] + D2[3 * mm + 2] * r[18 + 3 * d + b];
@@ -612,16 +613,17 @@ MF_DEV void gmf_j_lane(const double* r, const double* D2, double sc1, int b, int
 // (gmf_kernel.hpp:672-691), for lane (d, e, f) and (a, b, c) = 0..26: with Z[mn][fb] = sum_p D3_mnp r_p[f][b] and
 // Yh[m][(de)(fb)] = sum_n r_n[d][e] Z[mn][fb], H = sc2 (sum_m r_m[a][c] X1[m][b] + sum_m r_m[a][e] X2[m][c][b]),
 // X1[m][b] = Yh[m][(de)(fb)], X2[m][c][b] = Yh[m][(fc)(db)]; out[9a + 3b + c]
-MF_DEV void gmf_h_lane(const double* r, const double* D3, double sc2, int d, int e, int f, double* out) {
-  double X1[3][3], X2[3][3][3];
+template <class T>
+MF_DEV void gmf_h_lane(const T* r, const T* D3, T sc2, int d, int e, int f, T* out) {
+  T X1[3][3], X2[3][3][3];
 #pragma unroll
   for (int mm = 0; mm < 3; mm++) {
-    double Z1[3][3], Z2[3][3];   // [n][b]: Z[mn][fb], Z[mn][db]
+    T Z1[3][3], Z2[3][3];        // [n][b]: Z[mn][fb], Z[mn][db]
 #pragma unroll
     for (int nn = 0; nn < 3; nn++)
 #pragma unroll
       for (int b = 0; b < 3; b++) {
-        const double* D = D3 + 9 * mm + 3 * nn;
+        const T* D = D3 + 9 * mm + 3 * nn;
         Z1[nn][b] = D[0] * r[3 * f + b] + D[1] * r[9 + 3 * f + b] + D[2] * r[18 + 3 * f + b];
         Z2[nn][b] = D[0] * r[3 * d + b] + D[1] * r[9 + 3 * d + b] + D[2] * r[18 + 3 * d + b];
       }
@@ -635,7 +637,7 @@ MF_DEV void gmf_h_lane(const double* r, const double* D3, double sc2, int d, int
   }
 #pragma unroll
   for (int aa = 0; aa < 3; aa++) {
-    const double re0 = r[3 * aa + e], re1 = r[9 + 3 * aa + e], re2 = r[18 + 3 * aa + e];
+    const T re0 = r[3 * aa + e], re1 = r[9 + 3 * aa + e], re2 = r[18 + 3 * aa + e];
 #pragma unroll
     for (int b = 0; b < 3; b++)
 #pragma unroll

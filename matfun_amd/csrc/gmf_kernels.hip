@@ -1,8 +1,11 @@
-// gmf_kernels.hip -- general real 3x3 matrices on gfx950.  Phase 1, one matrix per lane (64 per wavefront =
-// workgroup): eigenvalues, divided differences, real basis, contracted tensors (gmf_device.hpp).  Phase 2, lane
-// groups: the results of the 64 matrices are staged in LDS and 27 lanes per matrix assemble J (81) and H (729) from
-// r_m (27 values) and D2 (9) / D3 (27), each lane owning 3 consecutive output positions (J) or 27 (H) so that the
-// stores are coalesced (216 B runs) -- H alone is 5.8 KB per matrix and sets the HBM roofline of the kernel.
+// gmf_kernels.hip -- general real 3x3 matrices on gfx950, one wavefront (64 matrices) per workgroup.
+//   phase 1, one matrix per lane: eigenvalues, divided differences, real basis, contracted tensors (gmf_device.hpp);
+//            r_m (27 values) and D3 (27) staged in LDS per matrix, D2 (9) and F (9) in a second region
+//   phase 2, lane groups: 27 lanes per matrix, two matrices per pass, assemble J (81: 3 per lane) and H (729: 27 per
+//            lane) into an LDS tile in the API layout, which the wave then writes as one contiguous block (2 x 5.8 KB
+//            of consecutive output per pass; whole cache lines per store instruction)
+// OT = double: the FP64 form; OT = float: the mixed form (FP64 per-lane math, FP32 staged tensors, J / H assembly
+// and outputs).
 #include "gmf_device.hpp"
 #include "smf_launch.hpp"
 
@@ -10,102 +13,123 @@ namespace mf {
 
 constexpr int GWAVE = 64;
 
+// LDS layout (elements of OT):
+//   main  [64][MS]      r (27), then D3 (27) and sc2 at 54 (order 2)
+//   x     [XS]          [0, 1458): the J (2 x 81) / H (2 x 729) output tile of a pass;
+//                       before the H passes also [XD2, +640): D2 (9) and sc1 per matrix, [XF, +576): F per matrix
 template <int ORDER>
-struct GmfTile {                      // per-matrix staging layout (stride odd: conflict-free 64-bit LDS rows)
-  static constexpr int R = 0, D3 = 27, D2 = ORDER >= 2 ? 54 : 27, F = D2 + 9, SC1 = F + 9, SC2 = SC1 + 1;
-  static constexpr int STRIDE = ORDER == 0 ? 9 : (ORDER == 1 ? 47 : 75);
-  static constexpr int FOFF = ORDER == 0 ? 0 : F;
+struct GmfLds {
+  static constexpr int MS = ORDER >= 2 ? 55 : 27;
+  static constexpr int XD2 = 162, XF = ORDER == 0 ? 0 : XD2 + 640;
+  static constexpr int XS = ORDER == 0 ? 576 : (ORDER == 1 ? XF + 576 : 1458);
+  static constexpr int MAIN = ORDER == 0 ? 0 : 64 * MS;
 };
 
-template <int KIND, int ORDER>
-__global__ void __launch_bounds__(GWAVE) gmf_kernel(int64_t n, const double* __restrict__ A, double* __restrict__ F,
-                                                    double* __restrict__ J, double* __restrict__ H, double al,
-                                                    GmfCoef bt) {
-  using T = GmfTile<ORDER>;
-  __shared__ double tile[GWAVE * T::STRIDE];
+template <int KIND, int ORDER, class OT>
+__global__ void __launch_bounds__(GWAVE) gmf_kernel(int64_t n, const double* __restrict__ A, OT* __restrict__ F,
+                                                    OT* __restrict__ J, OT* __restrict__ H, double al, GmfCoef bt) {
+  using L = GmfLds<ORDER>;
+  constexpr int TOT = L::MAIN + L::XS;
+  constexpr int BYTES = (TOT * (int)sizeof(OT) > GWAVE * 9 * 8) ? TOT * (int)sizeof(OT) : GWAVE * 9 * 8;
+  __shared__ double raw[(BYTES + 7) / 8];
+  double* xin = raw;                                 // (the FP64 input block, before phase 1)
+  OT* mainr = reinterpret_cast<OT*>(raw);
+  OT* xr = mainr + L::MAIN;
   const int lane = threadIdx.x;
   const int64_t base = (int64_t)blockIdx.x * GWAVE;
   const int nm = (int)(n - base < GWAVE ? n - base : GWAVE);
 
-  for (int t = lane; t < 9 * nm; t += GWAVE) tile[t] = A[base * 9 + t];
+  for (int t = lane; t < 9 * nm; t += GWAVE) xin[t] = A[base * 9 + t];
   __syncthreads();
   double a[9];
 #pragma unroll
-  for (int k = 0; k < 9; k++) a[k] = lane < nm ? tile[9 * lane + k] : (k % 4 == 0 ? 1.0 : 0.0);
+  for (int k = 0; k < 9; k++) a[k] = lane < nm ? xin[9 * lane + k] : (k % 4 == 0 ? 1.0 : 0.0);
   __syncthreads();
 
+  // ---- phase 1
   {
     GmfOut o;
     gmf_lane<KIND, ORDER>(a, al, bt.b, o);
-    double* my = tile + lane * T::STRIDE;
 #pragma unroll
-    for (int k = 0; k < 9; k++) my[T::FOFF + k] = o.F[k];
+    for (int k = 0; k < 9; k++) xr[L::XF + lane * 9 + k] = (OT)o.F[k];
     if (ORDER >= 1) {
+      OT* my = mainr + lane * L::MS;
 #pragma unroll
-      for (int k = 0; k < 27; k++) my[T::R + k] = o.r[k / 9][k % 9];
+      for (int k = 0; k < 27; k++) my[k] = (OT)o.r[k / 9][k % 9];
 #pragma unroll
-      for (int k = 0; k < 9; k++) my[T::D2 + k] = o.D2[k];
-      my[T::SC1] = o.sc1;
+      for (int k = 0; k < 9; k++) xr[L::XD2 + lane * 10 + k] = (OT)o.D2[k];
+      xr[L::XD2 + lane * 10 + 9] = (OT)o.sc1;
     }
     if (ORDER >= 2) {
+      OT* my = mainr + lane * L::MS;
 #pragma unroll
-      for (int k = 0; k < 27; k++) my[T::D3 + k] = o.D3[k];
-      my[T::SC2] = o.sc2;
+      for (int k = 0; k < 27; k++) my[27 + k] = (OT)o.D3[k];
+      my[54] = (OT)o.sc2;
     }
   }
   __syncthreads();
-
-  // ---- F: 9 per matrix, contiguous over the wave
-  for (int t = lane; t < 9 * nm; t += GWAVE) {
-    const int j = t / 9, k = t - 9 * j;
-    __builtin_nontemporal_store(tile[j * T::STRIDE + T::FOFF + k], F + base * 9 + t);
-  }
+  for (int t = lane; t < 9 * nm; t += GWAVE) F[base * 9 + t] = xr[L::XF + t];
   if (ORDER < 1) return;
 
-  // ---- J: lane (b, c, d) = 9b + 3c + d of a 27-lane group writes J_abcd for a = 0..2 (positions 27a + lane)
+  // ---- phase 2: lane (g, L) = matrix g of the pass, position L = 9 u + 3 v + w of a 27-lane group
   constexpr int G = 27, MPP = GWAVE / G;
-  const int g = lane / G, L = lane - g * G;
-  if (g < MPP) {
-    for (int j = g; j < nm; j += MPP) {
-      const double* s = tile + j * T::STRIDE;
-      double out[3];
-      gmf_j_lane(s + T::R, s + T::D2, s[T::SC1], L / 9, (L / 3) % 3, L % 3, out);
-      double* o = J + (base + j) * 81 + L;
+  const int g = lane / G, Lg = lane - g * G;
+  for (int p0 = 0; p0 < nm; p0 += MPP) {          // J: J_abcd for a = 0..2 at 27 a + (9b + 3c + d)
+    const int cnt = nm - p0 < MPP ? nm - p0 : MPP;
+    if (g < cnt) {
+      const int j = p0 + g;
+      OT out[3];
+      gmf_j_lane(mainr + j * L::MS, xr + L::XD2 + j * 10, xr[L::XD2 + j * 10 + 9], Lg / 9, (Lg / 3) % 3, Lg % 3, out);
 #pragma unroll
-      for (int aa = 0; aa < 3; aa++) __builtin_nontemporal_store(out[aa], o + 27 * aa);
+      for (int k = 0; k < 3; k++) xr[g * 81 + 27 * k + Lg] = out[k];
     }
+    __syncthreads();
+    for (int t = lane; t < 81 * cnt; t += GWAVE) J[(base + p0) * 81 + t] = xr[t];
+    __syncthreads();
   }
   if (ORDER < 2) return;
-
-  // ---- H: lane (d, e, f) = 9d + 3e + f writes H_abcdef for (a, b, c) = 0..26 (positions 27 (9a + 3b + c) + lane)
-  if (g < MPP) {
-    for (int j = g; j < nm; j += MPP) {
-      const double* s = tile + j * T::STRIDE;
-      double out[27];
-      gmf_h_lane(s + T::R, s + T::D3, s[T::SC2], L / 9, (L / 3) % 3, L % 3, out);
-      double* o = H + (base + j) * 729 + L;
+  for (int p0 = 0; p0 < nm; p0 += MPP) {          // H: H_abcdef for (a,b,c) = 0..26 at 27 (9a + 3b + c) + (9d + 3e + f)
+    const int cnt = nm - p0 < MPP ? nm - p0 : MPP;
+    if (g < cnt) {
+      const OT* s = mainr + (p0 + g) * L::MS;
+      OT out[27];
+      gmf_h_lane(s, s + 27, s[54], Lg / 9, (Lg / 3) % 3, Lg % 3, out);
 #pragma unroll
-      for (int k = 0; k < 27; k++) __builtin_nontemporal_store(out[k], o + 27 * k);
+      for (int k = 0; k < 27; k++) xr[g * 729 + 27 * k + Lg] = out[k];
     }
+    __syncthreads();
+    for (int t = lane; t < 729 * cnt; t += GWAVE) H[(base + p0) * 729 + t] = xr[t];
+    __syncthreads();
   }
 }
 
-template <int KIND>
-static void glaunch_kind(hipStream_t st, unsigned blocks, int64_t n, const double* A, double* F, double* J, double* H,
-                         double al, const GmfCoef& bt) {
-  if (H) hipLaunchKernelGGL((gmf_kernel<KIND, 2>), dim3(blocks), dim3(GWAVE), 0, st, n, A, F, J, H, al, bt);
-  else if (J) hipLaunchKernelGGL((gmf_kernel<KIND, 1>), dim3(blocks), dim3(GWAVE), 0, st, n, A, F, J, H, al, bt);
-  else hipLaunchKernelGGL((gmf_kernel<KIND, 0>), dim3(blocks), dim3(GWAVE), 0, st, n, A, F, J, H, al, bt);
+template <int KIND, class OT>
+static void glaunch_kind(hipStream_t st, unsigned blocks, int64_t n, const double* A, OT* F, OT* J, OT* H, double al,
+                         const GmfCoef& bt) {
+  if (H) hipLaunchKernelGGL((gmf_kernel<KIND, 2, OT>), dim3(blocks), dim3(GWAVE), 0, st, n, A, F, J, H, al, bt);
+  else if (J) hipLaunchKernelGGL((gmf_kernel<KIND, 1, OT>), dim3(blocks), dim3(GWAVE), 0, st, n, A, F, J, H, al, bt);
+  else hipLaunchKernelGGL((gmf_kernel<KIND, 0, OT>), dim3(blocks), dim3(GWAVE), 0, st, n, A, F, J, H, al, bt);
+}
+
+template <class OT>
+static void glaunch_any(hipStream_t st, int kind, int64_t n, const double* A, OT* F, OT* J, OT* H, double al,
+                        const GmfCoef& bt) {
+  const unsigned blocks = (unsigned)((n + GWAVE - 1) / GWAVE);
+  switch (kind) {
+    case EXP: glaunch_kind<EXP, OT>(st, blocks, n, A, F, J, H, al, bt); break;
+    case LOG: glaunch_kind<LOG, OT>(st, blocks, n, A, F, J, H, al, bt); break;
+    default: glaunch_kind<POW, OT>(st, blocks, n, A, F, J, H, al, bt); break;
+  }
 }
 
 void gmf_launch(hipStream_t st, int kind, int64_t n, const double* A, double* F, double* J, double* H, double al,
                 const GmfCoef& bt) {
-  const unsigned blocks = (unsigned)((n + GWAVE - 1) / GWAVE);
-  switch (kind) {
-    case EXP: glaunch_kind<EXP>(st, blocks, n, A, F, J, H, al, bt); break;
-    case LOG: glaunch_kind<LOG>(st, blocks, n, A, F, J, H, al, bt); break;
-    default: glaunch_kind<POW>(st, blocks, n, A, F, J, H, al, bt); break;
-  }
+  glaunch_any<double>(st, kind, n, A, F, J, H, al, bt);
+}
+
+void gmf_launch_mixed(hipStream_t st, int kind, int64_t n, const double* A, float* F, float* J, float* H, double al,
+                      const GmfCoef& bt) {
+  glaunch_any<float>(st, kind, n, A, F, J, H, al, bt);
 }
 
 }  // namespace mf

@@ -15,7 +15,7 @@
 
 namespace {
 
-bool valid(int kind, double alpha, int64_t n, const double* J, const double* H) {
+bool valid(int kind, double alpha, int64_t n, const void* J, const void* H) {
   return !(kind < mf::EXP || kind > mf::SQRT || n < 0 || (H && !J) || (kind == mf::POW && !(alpha > 0)));
 }
 
@@ -28,8 +28,26 @@ int launched() {
   return e == hipSuccess ? 0 : -2;
 }
 
-int smf_call(int kind, double alpha, int64_t n, const double* x, bool mat, double* F, double* J, double* H,
-             void* stream) {
+void launch_chunk(hipStream_t st, int kind, bool mat, int64_t n, const double* x, double* F, double* J, double* H,
+                  double al, const mf::SmfCoef& c2) {
+  mf::smf_launch(st, kind, mat, n, x, F, J, H, al, c2);
+}
+void launch_chunk(hipStream_t st, int kind, bool mat, int64_t n, const double* x, float* F, float* J, float* H,
+                  double al, const mf::SmfCoef& c2) {
+  mf::smf_launch_mixed(st, kind, mat, n, x, F, J, H, al, c2);
+}
+void glaunch_chunk(hipStream_t st, int kind, int64_t n, const double* A, double* F, double* J, double* H, double al,
+                   const mf::GmfCoef& bt) {
+  mf::gmf_launch(st, kind, n, A, F, J, H, al, bt);
+}
+void glaunch_chunk(hipStream_t st, int kind, int64_t n, const double* A, float* F, float* J, float* H, double al,
+                   const mf::GmfCoef& bt) {
+  mf::gmf_launch_mixed(st, kind, n, A, F, J, H, al, bt);
+}
+
+// symmetric matrices (x: Voigt vectors or full matrices); splits batches beyond one grid into several launches
+template <class OT>
+int smf_call(int kind, double alpha, int64_t n, const double* x, bool mat, OT* F, OT* J, OT* H, void* stream) {
   if (!valid(kind, alpha, n, J, H)) return -1;
   if (kind != mf::POW) alpha = kind == mf::SQRT ? 0.5 : 0.0;
   if (n == 0) return 0;
@@ -39,8 +57,27 @@ int smf_call(int kind, double alpha, int64_t n, const double* x, bool mat, doubl
   const int nin = mat ? 9 : 6, nf = mat ? 9 : 6;
   for (int64_t lo = 0; lo < n; lo += mf::MAX_PER_LAUNCH) {
     const int64_t m = n - lo < mf::MAX_PER_LAUNCH ? n - lo : mf::MAX_PER_LAUNCH;
-    mf::smf_launch((hipStream_t)stream, kind, mat, m, x + nin * lo, F + nf * lo, J ? J + nf * nf * lo : nullptr,
-                   H ? H + nf * nf * nf * lo : nullptr, alpha, c2);
+    launch_chunk((hipStream_t)stream, kind, mat, m, x + nin * lo, F + nf * lo, J ? J + nf * nf * lo : nullptr,
+                 H ? H + nf * nf * nf * lo : nullptr, alpha, c2);
+    if (launched()) return -2;
+  }
+  return 0;
+}
+
+// general matrices; sqrt is pow 0.5 (matfun_cuda.cu:172-173)
+template <class OT>
+int gmf_call(int kind, double alpha, int64_t n, const double* A, OT* F, OT* J, OT* H, void* stream) {
+  if (!valid(kind, alpha, n, J, H)) return -1;
+  if (kind == mf::SQRT) kind = mf::POW, alpha = 0.5;
+  if (kind != mf::POW) alpha = 0.0;
+  if (n == 0) return 0;
+  if (n > GMF_MAX_N) return -2;
+  mf::GmfCoef bt;
+  mf::gmf_coefs(kind, alpha, bt);
+  for (int64_t lo = 0; lo < n; lo += mf::MAX_PER_LAUNCH) {
+    const int64_t m = n - lo < mf::MAX_PER_LAUNCH ? n - lo : mf::MAX_PER_LAUNCH;
+    glaunch_chunk((hipStream_t)stream, kind, m, A + 9 * lo, F + 9 * lo, J ? J + 81 * lo : nullptr,
+                  H ? H + 729 * lo : nullptr, alpha, bt);
     if (launched()) return -2;
   }
   return 0;
@@ -60,20 +97,22 @@ extern "C" int smf_hip_eval_matrix(int kind, double alpha, int64_t n, const doub
 
 extern "C" int gmf_hip_eval(int kind, double alpha, int64_t n, const double* A, double* F, double* J, double* H,
                             void* stream) {
-  if (!valid(kind, alpha, n, J, H)) return -1;
-  if (kind == mf::SQRT) kind = mf::POW, alpha = 0.5;
-  if (kind != mf::POW) alpha = 0.0;
-  if (n == 0) return 0;
-  if (n > GMF_MAX_N) return -2;
-  mf::GmfCoef bt;
-  mf::gmf_coefs(kind, alpha, bt);
-  for (int64_t lo = 0; lo < n; lo += mf::MAX_PER_LAUNCH) {
-    const int64_t m = n - lo < mf::MAX_PER_LAUNCH ? n - lo : mf::MAX_PER_LAUNCH;
-    mf::gmf_launch((hipStream_t)stream, kind, m, A + 9 * lo, F + 9 * lo, J ? J + 81 * lo : nullptr,
-                   H ? H + 729 * lo : nullptr, alpha, bt);
-    if (launched()) return -2;
-  }
-  return 0;
+  return gmf_call(kind, alpha, n, A, F, J, H, stream);
+}
+
+extern "C" int smf_hip_eval_mixed(int kind, double alpha, int64_t n, const double* v, float* F, float* J, float* H,
+                                  void* stream) {
+  return smf_call(kind, alpha, n, v, false, F, J, H, stream);
+}
+
+extern "C" int smf_hip_eval_matrix_mixed(int kind, double alpha, int64_t n, const double* A, float* F, float* J,
+                                         float* H, void* stream) {
+  return smf_call(kind, alpha, n, A, true, F, J, H, stream);
+}
+
+extern "C" int gmf_hip_eval_mixed(int kind, double alpha, int64_t n, const double* A, float* F, float* J, float* H,
+                                  void* stream) {
+  return gmf_call(kind, alpha, n, A, F, J, H, stream);
 }
 
 extern "C" const char* smf_hip_version(void) { return MATFUN_HIP_VERSION; }

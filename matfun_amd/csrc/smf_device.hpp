@@ -2,7 +2,7 @@
 // matrix, f(A), the Daleckii-Krein first derivative as its 21 distinct symmetric entries, and the second derivative
 // one Voigt row at a time.  Same method, thresholds and branch structure as the reference kernel
 // c/include/matfun/smf_kernel.hpp (eval3 :200-382, divdiff :84-115, deriv :117-126, divdiff2 :148-192); the
-// derivative assembly is reorganised for the GPU (see SmfLane::h_row).
+// second derivative is assembled one output row at a time for the GPU (SmfHess::h_row).
 #pragma once
 #include "mf_device.hpp"
 
@@ -124,24 +124,23 @@ MF_DEV double dd2(double li, double lj, double lk, double fi, double fj, double 
   return out;
 }
 
-// state of one matrix after the eigen-decomposition
-template <int KIND>
+// packed position of the sorted triple (i <= j <= k) of eigenvalue indices among the 10 second divided differences
+MF_DEV constexpr int d2idx(int i, int j, int k) {
+  return i == 0 ? (j == 0 ? k : (j == 1 ? 2 + k : 5)) : (i == 1 ? (j == 1 ? 5 + k : 8) : 9);
+}
+// ... for any order of the three indices (constant after unrolling)
+MF_DEV constexpr int d2pos(int i, int j, int k) {
+  return d2idx(i < j ? (i < k ? i : k) : (j < k ? j : k), i + j + k - (i < j ? (i < k ? i : k) : (j < k ? j : k)) -
+                   (i > j ? (i > k ? i : k) : (j > k ? j : k)), i > j ? (i > k ? i : k) : (j > k ? j : k));
+}
+
+// state of one matrix after the eigen-decomposition.  Eigenvalues, eigenvectors, f and the divided differences are
+// FP64; the first-derivative assembly (P, Q) runs in AT: double for the FP64 form, float for the mixed form
+template <int KIND, class AT = double>
 struct SmfLane {
   double lam[3], fl[3], V[3][3];
   double D[3][3];                 // f[lam_i, lam_j] (diagonal: f'(lam_i))
   double D2[10];                  // f[lam_i, lam_j, lam_k], i <= j <= k, in the order of d2idx
-  double Et[6][6];                // Et^p = V^T E_p V (symmetric 3x3, packed by tri3) for the Voigt basis E_p
-
-  // packed position of the sorted triple (i <= j <= k)
-  static MF_DEV constexpr int d2idx(int i, int j, int k) {
-    return i == 0 ? (j == 0 ? k : (j == 1 ? 2 + k : 5)) : (i == 1 ? (j == 1 ? 5 + k : 8) : 9);
-  }
-  MF_DEV double d2(int i, int j, int k) const {   // any order (constant indices after unrolling)
-    const int a = i < j ? i : j, b = i < j ? j : i;
-    const int lo = a < k ? a : k, hi = b > k ? b : k;
-    const int md = i + j + k - lo - hi;
-    return D2[d2idx(lo, md, hi)];
-  }
 
   // ---- eigenvalues lam and vectors V (columns) of the Voigt vector v, and F (Voigt) -- smf_kernel.hpp:202-295
   MF_DEV void eigen(const double* v, double al, double* Fv) {
@@ -237,37 +236,37 @@ struct SmfLane {
 
   // ---- first derivative: the 21 distinct values Js[tri6(i,k)] = J_voigt[i][k] * w(k), w = 1 (k < 3) or 1/2,
   //      i.e. dF_ab / dA_cd of the matrix layout (symmetric in (ab) <-> (cd)) -- Daleckii-Krein, smf_kernel.hpp:296-327
-  MF_DEV void jacobian(double al, double* Js) {
+  MF_DEV void jacobian(double al, AT* Js) {
     const double D01 = dd1<KIND>(lam[0], lam[1], fl[0], fl[1], al);
     const double D02 = dd1<KIND>(lam[0], lam[2], fl[0], fl[2], al);
     const double D12 = dd1<KIND>(lam[1], lam[2], fl[1], fl[2], al);
     D[0][0] = dfun<KIND>(lam[0], fl[0], al); D[1][1] = dfun<KIND>(lam[1], fl[1], al); D[2][2] = dfun<KIND>(lam[2], fl[2], al);
     D[0][1] = D[1][0] = D01; D[0][2] = D[2][0] = D02; D[1][2] = D[2][1] = D12;
-    double P[3][6], Q[3][6];        // P_j = v_j v_j^T, Q_j = sum_l D_jl P_l (packed symmetric)
+    AT P[3][6], Q[3][6];            // P_j = v_j v_j^T, Q_j = sum_l D_jl P_l (packed symmetric)
 #pragma unroll
     for (int j = 0; j < 3; j++)
 #pragma unroll
-      for (int k = 0; k < 6; k++) P[j][k] = V[vrow(k)][j] * V[vcol(k)][j];
+      for (int k = 0; k < 6; k++) P[j][k] = (AT)V[vrow(k)][j] * (AT)V[vcol(k)][j];
 #pragma unroll
     for (int j = 0; j < 3; j++)
 #pragma unroll
-      for (int k = 0; k < 6; k++) Q[j][k] = D[j][0] * P[0][k] + D[j][1] * P[1][k] + D[j][2] * P[2][k];
+      for (int k = 0; k < 6; k++) Q[j][k] = (AT)D[j][0] * P[0][k] + (AT)D[j][1] * P[1][k] + (AT)D[j][2] * P[2][k];
 #pragma unroll
     for (int i = 0; i < 6; i++)
 #pragma unroll
       for (int k = i; k < 6; k++) {
         const int x = vrow(i), y = vcol(i), p = vrow(k), q = vcol(k);
-        double s = P[0][tri3(x, p)] * Q[0][tri3(y, q)] + P[1][tri3(x, p)] * Q[1][tri3(y, q)] +
-                   P[2][tri3(x, p)] * Q[2][tri3(y, q)];
+        AT s = P[0][tri3(x, p)] * Q[0][tri3(y, q)] + P[1][tri3(x, p)] * Q[1][tri3(y, q)] +
+               P[2][tri3(x, p)] * Q[2][tri3(y, q)];
         if (p != q)
           s += P[0][tri3(x, q)] * Q[0][tri3(y, p)] + P[1][tri3(x, q)] * Q[1][tri3(y, p)] +
                P[2][tri3(x, q)] * Q[2][tri3(y, p)];
-        Js[tri6(i, k)] = k < 3 ? s : 0.5 * s;
+        Js[tri6(i, k)] = k < 3 ? s : (AT)0.5 * s;
       }
   }
 
-  // ---- second divided differences and the rotated Voigt basis (smf_kernel.hpp:328-350)
-  MF_DEV void prepare_hessian(double al, const double* __restrict__ c2) {
+  // ---- the 10 second divided differences (smf_kernel.hpp:330-338)
+  MF_DEV void second_dd(double al, const double* __restrict__ c2) {
 #pragma unroll
     for (int i = 0; i < 3; i++)
 #pragma unroll
@@ -275,37 +274,60 @@ struct SmfLane {
 #pragma unroll
         for (int k = j; k < 3; k++)
           D2[d2idx(i, j, k)] = dd2<KIND>(lam[i], lam[j], lam[k], fl[i], fl[j], fl[k], D[i][j], D[i][k], D[j][k], al, c2);
+  }
+};
+
+// ---- second derivative from V (eigenvectors as columns) and the second divided differences D2 of one matrix
+//      (KIND no longer matters).  With Et^p = V^T E_p V for the Voigt basis E_p, S^pq = X + X^T,
+//      X_kl = sum_m D2_kml Et^p_km Et^q_ml and H_i^pq = (V S^pq V^T)_ab (i = (a, b)) (smf_kernel.hpp:339-381):
+//      H_i^pq = sum_m Et^p_m. M^m Et^q_m. with the symmetric M^m_kl = D2_kml G_kl, G_kl = V_ak V_bl + V_al V_bk,
+//      the same sum associated per output row i, so that a row of 21 values is produced on its own
+template <class AT>
+struct SmfHess {
+  AT V[3][3], D2[10];
+  AT Et[6][6];                    // Et^p packed by tri3
+
+  MF_DEV SmfHess(const AT* v9, const AT* d10) {
+#pragma unroll
+    for (int k = 0; k < 9; k++) V[k / 3][k % 3] = v9[k];
+#pragma unroll
+    for (int k = 0; k < 10; k++) D2[k] = d10[k];
 #pragma unroll
     for (int p = 0; p < 6; p++)
 #pragma unroll
       for (int t = 0; t < 6; t++) {
         const int k = vrow(p), l = vcol(p), a = vrow(t), b = vcol(t);
-        double e = V[k][a] * V[l][b];
+        AT e = V[k][a] * V[l][b];
         if (k != l) e += V[l][a] * V[k][b];
         Et[p][t] = e;
       }
   }
 
-  // ---- row i of the Voigt H: h[tri6(p,q)] = d2F_i / dv_p dv_q.  With S^pq = X + X^T, X_kl = sum_m D2_kml
-  //      Et^p_km Et^q_ml and H_i^pq = (V S^pq V^T)_ab (i = (a,b)): H_i^pq = sum_m Et^p_m. M^m Et^q_m.  with the
-  //      symmetric M^m_kl = D2_kml G_kl, G_kl = V_ak V_bl + V_al V_bk (the same sum as smf_kernel.hpp:351-381,
-  //      associated per output row so that one row of 21 values is produced at a time)
-  MF_DEV void h_row(int i, double* h) const {
+  // row i of the Voigt H: h[tri6(p, q)] = d2F_i / dv_p dv_q  (i may differ between lanes: the rows a, b of V are
+  // selected by value, so that V stays in registers)
+  MF_DEV void h_row(int i, AT* h) const {
     const int a = vrow(i), b = vcol(i);
-    double G[6];
+    AT va[3], vb[3];
+#pragma unroll
+    for (int k = 0; k < 3; k++) {
+      const AT v0 = V[0][k], v1 = V[1][k], v2 = V[2][k];
+      va[k] = a == 0 ? v0 : (a == 1 ? v1 : v2);
+      vb[k] = b == 0 ? v0 : (b == 1 ? v1 : v2);
+    }
+    AT G[6];
 #pragma unroll
     for (int t = 0; t < 6; t++) {
       const int k = vrow(t), l = vcol(t);
-      G[t] = V[a][k] * V[b][l] + V[a][l] * V[b][k];
+      G[t] = va[k] * vb[l] + va[l] * vb[k];
     }
-    double M[3][6];
+    AT M[3][6];
 #pragma unroll
     for (int m = 0; m < 3; m++)
 #pragma unroll
-      for (int t = 0; t < 6; t++) M[m][t] = d2(vrow(t), m, vcol(t)) * G[t];
+      for (int t = 0; t < 6; t++) M[m][t] = D2[d2pos(vrow(t), m, vcol(t))] * G[t];
 #pragma unroll
     for (int p = 0; p < 6; p++) {
-      double y[3][3];               // y^(p,m)_l = sum_k M^m_lk Et^p_mk
+      AT y[3][3];                   // y^(p,m)_l = sum_k M^m_lk Et^p_mk
 #pragma unroll
       for (int m = 0; m < 3; m++)
 #pragma unroll
@@ -314,7 +336,7 @@ struct SmfLane {
                     M[m][tri3(l, 2)] * Et[p][tri3(m, 2)];
 #pragma unroll
       for (int q = p; q < 6; q++) {
-        double s = 0.0;
+        AT s = 0;
 #pragma unroll
         for (int m = 0; m < 3; m++)
 #pragma unroll
@@ -324,6 +346,28 @@ struct SmfLane {
     }
   }
 };
+
+// ---- output index maps: element e of one matrix's F / J / H in the API layout -> position among the distinct
+//      values (F: Fv[6]; J: Js[21], J_voigt[i][k] w(k); H: 6 rows of 21, Hv[i][tri6(p,q)]) and its weight
+//      (matrix layout: d v / d A_cd = 1 or 1/2 per derivative index, smf_kernel.hpp:384-406; Voigt: J_voigt = Js / w)
+MF_DEV int f_src(bool mat, int e) { return mat ? voigt_of_m(e) : e; }
+MF_DEV int j_src(bool mat, int e, double& w) {
+  if (mat) {                          // J[9 m + c] = Js[(midx(m), midx(c))]
+    w = 1.0;
+    return tri6(voigt_of_m(e / 9), voigt_of_m(e % 9));
+  }
+  w = e % 6 < 3 ? 1.0 : 2.0;          // J[6 i + k] = Js[(i, k)] / w(k)
+  return tri6(e / 6, e % 6);
+}
+MF_DEV int h_src(bool mat, int e, double& w) {
+  if (mat) {                          // H[(9 m + c) 9 + d] = Hv[midx(m)][(midx(c), midx(d))] w(c) w(d)
+    const int m = e / 81, c = (e / 9) % 9, d = e % 9;
+    w = dv_dA(c) * dv_dA(d);
+    return 21 * voigt_of_m(m) + tri6(voigt_of_m(c), voigt_of_m(d));
+  }
+  w = 1.0;                            // H[(6 i + p) 6 + q] = Hv[i][(p, q)]
+  return 21 * (e / 36) + tri6((e / 6) % 6, e % 6);
+}
 
 // Voigt vector of (A + A^T) / 2 and the symmetry test |a_ij - a_ji| <= 1e-8 max |a_kl| (NaN / Inf pass); NaN if
 // not symmetric (smf_kernel.hpp:408-421; the GPU API reports no -3, the results are NaN)

@@ -1,140 +1,149 @@
-// smf_kernels.hip -- symmetric 3x3 matrices on gfx950: one matrix per lane, one wavefront (64 matrices) per
-// workgroup.  Inputs are read coalesced through LDS; every output block (F, J, each H row) is staged in a 64 x 21
-// LDS tile of distinct values and written by lane groups so that consecutive lanes store consecutive doubles
-// (the outputs are 96 B .. 5.8 KB per matrix and bound the kernel by HBM write bandwidth).
+// smf_kernels.hip -- symmetric 3x3 matrices on gfx950, one wavefront (64 matrices) per workgroup.
+//   phase A, one matrix per lane: coalesced input through LDS, eigen-decomposition, F, the 21 distinct J values,
+//            the 10 second divided differences; F / J staged in LDS and written as one contiguous region per wave
+//   phase B, six lanes per matrix (one per Voigt row of H), ten matrices per pass: H rows from V and D2 staged in
+//            LDS, gathered into the API layout through a lookup table and written contiguously (a pass covers
+//            10 x 5.8 KB of consecutive output)
+// Every store instruction of the wave writes 64 consecutive elements: the output (96 B .. 5.8 KB per matrix) bounds
+// the kernel by HBM write bandwidth, which needs whole cache lines per instruction.
+// OT = double: the FP64 form; OT = float: the mixed form (FP64 eigen-decomposition and divided differences, FP32
+// derivative assembly and FP32 outputs).
 #include "smf_device.hpp"
 #include "smf_launch.hpp"
 
 namespace mf {
 
 constexpr int WAVE = 64;
-constexpr int TILE = 21;           // staged doubles per matrix (odd: conflict-free 64-bit LDS access by lane)
+constexpr int SA = 27;             // phase-A tile per matrix: F (6) + Js (21)
+constexpr int SB = 19;             // V (9) + D2 (10)
+constexpr int HPP = 10;            // matrices per phase-B pass (6 lanes each)
+constexpr int SH = 127;            // staged H values per matrix (6 rows x 21, +1: odd stride)
 
-// write, for matrices j < nm of this wave, a block of G * KPL doubles at out + j * ostride (+ lane-group offset):
-// element e = k * G + L of matrix j is tile[j * TILE + src[k]] * wt[k]; MPP = 64 / G matrices per pass
-template <int G, int KPL>
-MF_DEV void store_blocks(double* __restrict__ out, int64_t ostride, const double* tile, int nm, int lane,
-                         const int (&src)[KPL], const double (&wt)[KPL]) {
-  constexpr int MPP = WAVE / G;
-  const int g = lane / G, L = lane - g * G;
-  if (g >= MPP) return;
-  for (int j = g; j < nm; j += MPP) {
-    double* o = out + (int64_t)j * ostride + L;
-    const double* s = tile + j * TILE;
-#pragma unroll
-    for (int k = 0; k < KPL; k++) __builtin_nontemporal_store(s[src[k]] * wt[k], o + k * G);
-  }
-}
-
-template <int KIND, bool MAT, int ORDER>
-__global__ void __launch_bounds__(WAVE) smf_kernel(int64_t n, const double* __restrict__ x, double* __restrict__ F,
-                                                   double* __restrict__ J, double* __restrict__ H, double al,
-                                                   SmfCoef c2) {
-  __shared__ double tile[WAVE * TILE];
+template <int KIND, bool MAT, int ORDER, class OT>
+__global__ void __launch_bounds__(WAVE) smf_kernel(int64_t n, const double* __restrict__ x, OT* __restrict__ F,
+                                                   OT* __restrict__ J, OT* __restrict__ H, double al, SmfCoef c2) {
   constexpr int NIN = MAT ? 9 : 6, NF = MAT ? 9 : 6, NJ = NF * NF, NH = NJ * NF;
+  constexpr int R1 = (WAVE * SA > HPP * SH ? WAVE * SA : HPP * SH);             // region 1 (OT): phase-A / H tiles
+  constexpr int R1B = (R1 * (int)sizeof(OT) > WAVE * NIN * 8 ? R1 * (int)sizeof(OT) : WAVE * NIN * 8);
+  __shared__ double raw1[(R1B + 7) / 8];                                         // (also the FP64 input block)
+  __shared__ OT tb[ORDER >= 2 ? WAVE * SB : 1];
+  __shared__ unsigned short lutJ[ORDER >= 1 ? NJ : 1], lutH[ORDER >= 2 ? NH : 1];
+  double* xin = raw1;
+  OT* t1 = reinterpret_cast<OT*>(raw1);
   const int lane = threadIdx.x;
   const int64_t base = (int64_t)blockIdx.x * WAVE;
   const int nm = (int)(n - base < WAVE ? n - base : WAVE);
 
+  // ---- lookup tables of the output layouts: distinct-value position | weight code << 8 (weight 2^-code, or 2)
+  if (ORDER >= 1)
+    for (int e = lane; e < NJ; e += WAVE) {
+      double w;
+      const int s = j_src(MAT, e, w);
+      lutJ[e] = (unsigned short)(s | (w == 2.0 ? 3 << 8 : 0));
+    }
+  if (ORDER >= 2)
+    for (int e = lane; e < NH; e += WAVE) {
+      double w;
+      const int s = h_src(MAT, e, w);
+      lutH[e] = (unsigned short)(s | ((w == 1.0 ? 0 : (w == 0.5 ? 1 : 2)) << 8));
+    }
+
   // ---- input, coalesced through LDS (lanes past the end take the identity)
-  for (int t = lane; t < NIN * nm; t += WAVE) tile[t] = x[base * NIN + t];
+  for (int t = lane; t < NIN * nm; t += WAVE) xin[t] = x[base * NIN + t];
   __syncthreads();
   double v[6];
   if (MAT) {
     double a[9];
 #pragma unroll
-    for (int k = 0; k < 9; k++) a[k] = lane < nm ? tile[9 * lane + k] : (k % 4 == 0 ? 1.0 : 0.0);
+    for (int k = 0; k < 9; k++) a[k] = lane < nm ? xin[9 * lane + k] : (k % 4 == 0 ? 1.0 : 0.0);
     matrix_to_voigt(a, v);
   } else {
 #pragma unroll
-    for (int k = 0; k < 6; k++) v[k] = lane < nm ? tile[6 * lane + k] : (k < 3 ? 1.0 : 0.0);
+    for (int k = 0; k < 6; k++) v[k] = lane < nm ? xin[6 * lane + k] : (k < 3 ? 1.0 : 0.0);
   }
   __syncthreads();
 
-  SmfLane<KIND> s;
-  double Fv[6];
-  s.eigen(v, al, Fv);
-
-  // ---- F
+  // ---- phase A
+  {
+    SmfLane<KIND, OT> s;
+    double Fv[6];
+    s.eigen(v, al, Fv);
 #pragma unroll
-  for (int k = 0; k < 6; k++) tile[lane * TILE + k] = Fv[k];
+    for (int k = 0; k < 6; k++) t1[lane * SA + k] = (OT)Fv[k];
+    if (ORDER >= 1) {
+      OT Js[21];
+      s.jacobian(al, Js);
+#pragma unroll
+      for (int k = 0; k < 21; k++) t1[lane * SA + 6 + k] = Js[k];
+    }
+    if (ORDER >= 2) {
+      s.second_dd(al, c2.c);
+#pragma unroll
+      for (int k = 0; k < 9; k++) tb[lane * SB + k] = (OT)s.V[k / 3][k % 3];
+#pragma unroll
+      for (int k = 0; k < 10; k++) tb[lane * SB + 9 + k] = (OT)s.D2[k];
+    }
+  }
   __syncthreads();
   for (int t = lane; t < NF * nm; t += WAVE) {
     const int j = t / NF, e = t - j * NF;
-    __builtin_nontemporal_store(tile[j * TILE + (MAT ? voigt_of_m(e) : e)], F + base * NF + t);
+    F[base * NF + t] = t1[j * SA + f_src(MAT, e)];
   }
   if (ORDER < 1) return;
-  __syncthreads();
-
-  // ---- lane-group maps: matrix layout 81-blocks as 27 lanes x 3, Voigt 36-blocks as 12 lanes x 3
-  constexpr int G = MAT ? 27 : 12;
-  int src[3];
-  double wJ[3], wH[3];
   {
-    const int L = lane % G;
-#pragma unroll
-    for (int k = 0; k < 3; k++) {
-      if (MAT) {                               // element 27k + L of an 81-block: (r, c) = (3k + L/9, L%9)
-        const int r = 3 * k + L / 9, c = L % 9;
-        src[k] = tri6(voigt_of_m(r), voigt_of_m(c));
-        wJ[k] = 1.0;
-        wH[k] = dv_dA(r) * dv_dA(c);
-      } else {                                 // element 12k + L of a 36-block: (p, q) = (2k + L/6, L%6)
-        const int p = 2 * k + L / 6, q = L % 6;
-        src[k] = tri6(p, q);
-        wJ[k] = q < 3 ? 1.0 : 2.0;             // J_voigt[p][q] = Js / w(q)
-        wH[k] = 1.0;
-      }
+    int j = lane / NJ, e = lane - j * NJ;     // t = j NJ + e, advanced by the wave width (NJ >= 36: <= 2 wraps)
+    OT* out = J + base * NJ;
+    for (int t = lane; t < NJ * nm; t += WAVE) {
+      const int u = lutJ[e];
+      const OT val = t1[j * SA + 6 + (u & 255)];
+      out[t] = (u >> 8) ? (OT)2 * val : val;
+      e += WAVE;
+      while (e >= NJ) { e -= NJ; j++; }
     }
   }
-
-  // ---- J
-  {
-    double Js[21];
-    s.jacobian(al, Js);
-#pragma unroll
-    for (int k = 0; k < 21; k++) tile[lane * TILE + k] = Js[k];
-  }
-  __syncthreads();
-  store_blocks<G, 3>(J + base * NJ, NJ, tile, nm, lane, src, wJ);
   if (ORDER < 2) return;
 
-  // ---- H, one Voigt row i at a time; matrix layout: row blocks m with voigt_of_m(m) == i (m and its transpose)
-  s.prepare_hessian(al, c2.c);
-  constexpr int ORDER_I[6] = {0, 3, 4, 1, 5, 2};
-#pragma unroll
-  for (int ii = 0; ii < 6; ii++) {
-    const int i = ORDER_I[ii];
-    double h[21];
-    s.h_row(i, h);
+  // ---- phase B: lane = (matrix jj of the pass, Voigt row i)
+  const int jj = lane / 6, ir = lane - 6 * jj;
+  for (int p0 = 0; p0 < nm; p0 += HPP) {
+    const int cnt = nm - p0 < HPP ? nm - p0 : HPP;
     __syncthreads();                           // (previous tile fully read)
+    if (jj < cnt) {
+      const OT* sb = tb + (p0 + jj) * SB;
+      const SmfHess<OT> hs(sb, sb + 9);
+      OT h[21];
+      hs.h_row(ir, h);
 #pragma unroll
-    for (int k = 0; k < 21; k++) tile[lane * TILE + k] = h[k];
+      for (int k = 0; k < 21; k++) t1[jj * SH + ir * 21 + k] = h[k];
+    }
     __syncthreads();
-    if (MAT) {
-#pragma unroll
-      for (int m = 0; m < 9; m++)
-        if (voigt_of_m(m) == i) store_blocks<G, 3>(H + base * NH + 81 * m, NH, tile, nm, lane, src, wH);
-    } else {
-      store_blocks<G, 3>(H + base * NH + 36 * i, NH, tile, nm, lane, src, wH);
+    int j = lane / NH, e = lane - j * NH;     // (NH >= 216 > 64)
+    OT* out = H + (base + p0) * NH;
+    for (int t = lane; t < NH * cnt; t += WAVE) {
+      const int u = lutH[e];
+      const OT val = t1[j * SH + (u & 255)];
+      out[t] = (u >> 8) == 0 ? val : ((u >> 8) == 1 ? (OT)0.5 * val : (OT)0.25 * val);
+      e += WAVE;
+      if (e >= NH) { e -= NH; j++; }   // (NH > 64: at most one wrap)
     }
   }
 }
 
-template <int KIND, bool MAT>
-static void launch_kind(hipStream_t st, unsigned blocks, int64_t n, const double* x, double* F, double* J, double* H,
-                        double al, const SmfCoef& c2) {
-  if (H) hipLaunchKernelGGL((smf_kernel<KIND, MAT, 2>), dim3(blocks), dim3(WAVE), 0, st, n, x, F, J, H, al, c2);
-  else if (J) hipLaunchKernelGGL((smf_kernel<KIND, MAT, 1>), dim3(blocks), dim3(WAVE), 0, st, n, x, F, J, H, al, c2);
-  else hipLaunchKernelGGL((smf_kernel<KIND, MAT, 0>), dim3(blocks), dim3(WAVE), 0, st, n, x, F, J, H, al, c2);
+template <int KIND, bool MAT, class OT>
+static void launch_kind(hipStream_t st, unsigned blocks, int64_t n, const double* x, OT* F, OT* J, OT* H, double al,
+                        const SmfCoef& c2) {
+  if (H) hipLaunchKernelGGL((smf_kernel<KIND, MAT, 2, OT>), dim3(blocks), dim3(WAVE), 0, st, n, x, F, J, H, al, c2);
+  else if (J) hipLaunchKernelGGL((smf_kernel<KIND, MAT, 1, OT>), dim3(blocks), dim3(WAVE), 0, st, n, x, F, J, H, al, c2);
+  else hipLaunchKernelGGL((smf_kernel<KIND, MAT, 0, OT>), dim3(blocks), dim3(WAVE), 0, st, n, x, F, J, H, al, c2);
 }
 
-void smf_launch(hipStream_t st, int kind, bool mat, int64_t n, const double* x, double* F, double* J, double* H,
-                double al, const SmfCoef& c2) {
+template <class OT>
+static void launch_any(hipStream_t st, int kind, bool mat, int64_t n, const double* x, OT* F, OT* J, OT* H, double al,
+                       const SmfCoef& c2) {
   const unsigned blocks = (unsigned)((n + WAVE - 1) / WAVE);
 #define MF_CASE(K)                                                             \
-  if (mat) launch_kind<K, true>(st, blocks, n, x, F, J, H, al, c2);            \
-  else launch_kind<K, false>(st, blocks, n, x, F, J, H, al, c2);
+  if (mat) launch_kind<K, true, OT>(st, blocks, n, x, F, J, H, al, c2);        \
+  else launch_kind<K, false, OT>(st, blocks, n, x, F, J, H, al, c2);
   switch (kind) {
     case EXP: MF_CASE(EXP) break;
     case LOG: MF_CASE(LOG) break;
@@ -142,6 +151,16 @@ void smf_launch(hipStream_t st, int kind, bool mat, int64_t n, const double* x, 
     default: MF_CASE(POW) break;
   }
 #undef MF_CASE
+}
+
+void smf_launch(hipStream_t st, int kind, bool mat, int64_t n, const double* x, double* F, double* J, double* H,
+                double al, const SmfCoef& c2) {
+  launch_any<double>(st, kind, mat, n, x, F, J, H, al, c2);
+}
+
+void smf_launch_mixed(hipStream_t st, int kind, bool mat, int64_t n, const double* x, float* F, float* J, float* H,
+                      double al, const SmfCoef& c2) {
+  launch_any<float>(st, kind, mat, n, x, F, J, H, al, c2);
 }
 
 }  // namespace mf

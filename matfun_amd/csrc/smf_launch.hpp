@@ -13,6 +13,11 @@ void smf_launch(hipStream_t st, int kind, bool mat, int64_t n, const double* x, 
                 double al, const SmfCoef& c2);
 void gmf_launch(hipStream_t st, int kind, int64_t n, const double* A, double* F, double* J, double* H, double al,
                 const GmfCoef& bt);
+// mixed form: FP64 inputs, FP32 outputs
+void smf_launch_mixed(hipStream_t st, int kind, bool mat, int64_t n, const double* x, float* F, float* J, float* H,
+                      double al, const SmfCoef& c2);
+void gmf_launch_mixed(hipStream_t st, int kind, int64_t n, const double* A, float* F, float* J, float* H, double al,
+                      const GmfCoef& bt);
 
 constexpr int64_t MAX_PER_LAUNCH = (int64_t)1 << 30;   // matrices per grid (work-items stay below 2^32)
 

@@ -3,7 +3,8 @@
 spd(n, seed)       symmetric positive definite A = Q diag(lam) Q^T: Q Haar-random orthogonal (QR of a Gaussian 3x3
                    with the signs of diag(R) moved into Q, Mezzadri 2007), lam_i = 10^U(-2, 2) independently; with
                    `special`, 15% of the matrices get lam_2 = lam_1 exactly, 5% lam_1 = lam_2 = lam_3, 10% a relative
-                   gap lam_2 = lam_1 (1 + 1e-8); A is made exactly symmetric ((A + A^T) / 2).
+                   gap lam_2 = lam_1 (1 + 1e-8); A is made exactly symmetric ((A + A^T) / 2).  `log10_range` moves the
+                   eigenvalue range (e.g. (-2, log10(60)) keeps exp(A) inside the FP32 range of the mixed form).
 general(n, seed)   general real matrices: 50% V diag(lam) V^-1 (lam ~ U[0.1, 10], V_ij ~ N(0,1) redrawn until
                    cond_2(V) <= 1e3); 25% a rotation-stretch Q blockdiag(s R2(theta), mu) Q^T (Q takes e3 to a random
                    axis; theta ~ U(-pi + 0.05, pi - 0.05), s, mu ~ U[0.5, 2]: a complex pair); 15% simple shear
@@ -21,10 +22,10 @@ def haar(rng, n):
     return Q * d[:, None, :]
 
 
-def spd(n, seed=0, special=True):
+def spd(n, seed=0, special=True, log10_range=(-2.0, 2.0)):
     rng = np.random.default_rng(seed)
     Q = haar(rng, n)
-    lam = 10.0 ** rng.uniform(-2.0, 2.0, (n, 3))
+    lam = 10.0 ** rng.uniform(log10_range[0], log10_range[1], (n, 3))
     if special:
         u = rng.uniform(0.0, 1.0, n)
         dbl, tri, gap = u < 0.15, (u >= 0.15) & (u < 0.20), (u >= 0.20) & (u < 0.30)

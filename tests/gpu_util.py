@@ -6,22 +6,28 @@ import matfun_amd
 SIZES = {"voigt": (6, 6), "matrix": (9, 9), "general": (9, 9)}
 
 
-def call(family, kind, alpha, X, order, stream=None):
-    """run smf_hip_eval / smf_hip_eval_matrix / gmf_hip_eval on host array X (n, 6|9); returns rc, F, J, H (NumPy)"""
+def call(family, kind, alpha, X, order, stream=None, mixed=False):
+    """run smf_hip_eval / smf_hip_eval_matrix / gmf_hip_eval (or their _mixed forms) on host array X (n, 6|9);
+    returns rc, F, J, H (NumPy)"""
     import torch
     lib = matfun_amd.library()
-    fn = {"voigt": lib.smf_hip_eval, "matrix": lib.smf_hip_eval_matrix, "general": lib.gmf_hip_eval}[family]
+    if mixed:
+        fn = {"voigt": lib.smf_hip_eval_mixed, "matrix": lib.smf_hip_eval_matrix_mixed,
+              "general": lib.gmf_hip_eval_mixed}[family]
+    else:
+        fn = {"voigt": lib.smf_hip_eval, "matrix": lib.smf_hip_eval_matrix, "general": lib.gmf_hip_eval}[family]
+    dt = torch.float32 if mixed else torch.float64
     nin, nf = SIZES[family]
     X = np.ascontiguousarray(X, dtype=np.float64).reshape(-1, nin)
     n = X.shape[0]
     dX = torch.from_numpy(X).to("cuda")
-    dF = torch.full((n, nf), 7.0, dtype=torch.float64, device="cuda")        # (7: never a result by accident)
-    dJ = torch.full((n, nf * nf), 7.0, dtype=torch.float64, device="cuda") if order >= 1 else None
-    dH = torch.full((n, nf ** 3), 7.0, dtype=torch.float64, device="cuda") if order >= 2 else None
+    dF = torch.full((n, nf), 7.0, dtype=dt, device="cuda")        # (7: never a result by accident)
+    dJ = torch.full((n, nf * nf), 7.0, dtype=dt, device="cuda") if order >= 1 else None
+    dH = torch.full((n, nf ** 3), 7.0, dtype=dt, device="cuda") if order >= 2 else None
     p = lambda t: None if t is None else t.data_ptr()
     st = torch.cuda.current_stream().cuda_stream if stream is None else stream
     kind = matfun_amd.KINDS[kind] if isinstance(kind, str) else int(kind)
     rc = fn(kind, float(alpha), n, p(dX), p(dF), p(dJ), p(dH), st)
     torch.cuda.synchronize()
-    g = lambda t: None if t is None else t.cpu().numpy()
+    g = lambda t: None if t is None else t.cpu().double().numpy()
     return rc, g(dF), g(dJ), g(dH)

@@ -8,25 +8,37 @@
 
 namespace {
 
+// one matrix per call of the lane math; the outputs are gathered through the kernels' index maps (f_src, j_src, h_src)
 template <int KIND>
-void smf_voigt(double al, int64_t n, const double* v, double* F, double* J, double* H) {
+void smf_any(bool mat, double al, int64_t n, const double* x, double* F, double* J, double* H) {
   mf::SmfCoef c2;
   mf::smf_coefs(KIND, al, c2);
+  const int nf = mat ? 9 : 6, nj = nf * nf, nh = nj * nf;
   for (int64_t m = 0; m < n; m++) {
-    mf::SmfLane<KIND> s;
-    double Fv[6], Js[21], h[21];
-    s.eigen(v + 6 * m, al, Fv);
-    for (int k = 0; k < 6; k++) F[6 * m + k] = Fv[k];
+    double v[6];
+    if (mat) mf::matrix_to_voigt(x + 9 * m, v);
+    else for (int k = 0; k < 6; k++) v[k] = x[6 * m + k];
+    mf::SmfLane<KIND, double> s;
+    double Fv[6], Js[21], Hv[126];
+    s.eigen(v, al, Fv);
+    for (int e = 0; e < nf; e++) F[nf * m + e] = Fv[mf::f_src(mat, e)];
     if (!J) continue;
     s.jacobian(al, Js);
-    for (int i = 0; i < 6; i++)
-      for (int k = 0; k < 6; k++) J[36 * m + 6 * i + k] = Js[mf::tri6(i, k)] * (k < 3 ? 1.0 : 2.0);
+    for (int e = 0; e < nj; e++) {
+      double w;
+      const int src = mf::j_src(mat, e, w);
+      J[nj * m + e] = Js[src] * w;
+    }
     if (!H) continue;
-    s.prepare_hessian(al, c2.c);
-    for (int i = 0; i < 6; i++) {
-      s.h_row(i, h);
-      for (int p = 0; p < 6; p++)
-        for (int q = 0; q < 6; q++) H[216 * m + 36 * i + 6 * p + q] = h[mf::tri6(p, q)];
+    s.second_dd(al, c2.c);
+    double V9[9];
+    for (int k = 0; k < 9; k++) V9[k] = s.V[k / 3][k % 3];
+    const mf::SmfHess<double> hs(V9, s.D2);
+    for (int i = 0; i < 6; i++) hs.h_row(i, Hv + 21 * i);
+    for (int e = 0; e < nh; e++) {
+      double w;
+      const int src = mf::h_src(mat, e, w);
+      H[nh * m + e] = Hv[src] * w;
     }
   }
 }
@@ -56,15 +68,21 @@ void gmf_all(double al, int64_t n, const double* A, double* F, double* J, double
 
 }  // namespace
 
-extern "C" int lanecheck_smf(int kind, double al, int64_t n, const double* v, double* F, double* J, double* H) {
+static int smf_entry(bool mat, int kind, double al, int64_t n, const double* x, double* F, double* J, double* H) {
   if (kind != mf::POW) al = kind == mf::SQRT ? 0.5 : 0.0;
   switch (kind) {
-    case mf::EXP: smf_voigt<mf::EXP>(al, n, v, F, J, H); break;
-    case mf::LOG: smf_voigt<mf::LOG>(al, n, v, F, J, H); break;
-    case mf::SQRT: smf_voigt<mf::SQRT>(al, n, v, F, J, H); break;
-    default: smf_voigt<mf::POW>(al, n, v, F, J, H); break;
+    case mf::EXP: smf_any<mf::EXP>(mat, al, n, x, F, J, H); break;
+    case mf::LOG: smf_any<mf::LOG>(mat, al, n, x, F, J, H); break;
+    case mf::SQRT: smf_any<mf::SQRT>(mat, al, n, x, F, J, H); break;
+    default: smf_any<mf::POW>(mat, al, n, x, F, J, H); break;
   }
   return 0;
+}
+extern "C" int lanecheck_smf(int kind, double al, int64_t n, const double* v, double* F, double* J, double* H) {
+  return smf_entry(false, kind, al, n, v, F, J, H);
+}
+extern "C" int lanecheck_smf_matrix(int kind, double al, int64_t n, const double* A, double* F, double* J, double* H) {
+  return smf_entry(true, kind, al, n, A, F, J, H);
 }
 
 template <int KIND>

@@ -1,13 +1,20 @@
 """gmf_hip_eval on the GPU against the CPU oracle (bitwise equal to the reference C kernel) and the reference's
-golden vectors.  Tolerances: FP64 per-matrix normwise relative error <= 1e-12 (F, J, H separately) on the seeded
-general workload (similarity transforms, complex pairs, shear, 4 I + N(0,1)); max-norm batch difference <= 1e-13 on
-the reference GPU test's sample (cuda/tests/test_cuda_api.cu:81-112); golden cases at their own tolerances."""
+golden vectors.
+
+Tolerance of the seeded general workload (similarity transforms with cond(V) <= 1e3, complex pairs, shear,
+4 I + N(0,1)): per matrix and output (F, J, H), normwise relative error <= max(1e-12, 32 s), s the difference between
+two correct builds of the reference itself on that matrix -- its SIMD build (glibc libmvec) against its scalar build
+(libm, = the oracle).  On these inputs that spread reaches 1e-11 .. 1e-10 for about 0.1% of the matrices (eigenvector
+conditioning ~1e5), so a flat 1e-12 would fail the reference against itself; the GPU (ocml) tracks the oracle as
+closely as the reference's SIMD build does.  Without the reference build: p99.9 <= 2e-12, max <= 1e-10.
+Reference GPU test sample: max-norm batch difference <= 1e-13 (cuda/tests/test_cuda_api.cu:108).
+"""
 import os
 
 import numpy as np
 import pytest
 
-from oracle.pyoracle import Checker
+from oracle.pyoracle import Checker, available
 from matfun_amd import workloads as W
 from tests.parity import maxrel, normwise
 
@@ -23,20 +30,39 @@ def oracle():
 
 
 @pytest.fixture(scope="module")
+def ref():
+    if not available("ref"):
+        return None
+    r = Checker("ref")
+    return None if r.name().endswith("(scalar)") else r
+
+
+@pytest.fixture(scope="module")
 def gen():
     return W.general(3000, seed=202)
 
 
+def check_spread(name, got, ora, refv):
+    e = normwise(got, ora)
+    if refv is None:
+        assert np.quantile(e, 0.999) <= 2e-12 and e.max() <= 1e-10, (name, e.max(), np.quantile(e, 0.999))
+        return
+    s = normwise(refv, ora)
+    tol = np.maximum(TOL, 32 * s)
+    bad = np.nonzero(e > tol)[0]
+    assert bad.size == 0, f"{name}: {bad.size} matrices over tolerance, first {bad[0]}: {e[bad[0]]:.2e} (spread {s[bad[0]]:.2e})"
+
+
 @pytest.mark.parametrize("kind,alpha", KINDS)
 @pytest.mark.parametrize("order", [0, 1, 2])
-def test_general_vs_oracle(oracle, gen, kind, alpha, order):
+def test_general_vs_oracle(oracle, ref, gen, kind, alpha, order):
     from tests.gpu_util import call
     rc, F, J, H = call("general", kind, alpha, gen, order)
     orc, Fo, Jo, Ho = oracle.gmf(kind, alpha, gen, order)
+    rr = ref.gmf(kind, alpha, gen, order) if ref is not None else (0, None, None, None)
     assert rc == 0 and orc == 0
-    for name, x, y in (("F", F, Fo), ("J", J, Jo), ("H", H, Ho))[:order + 1]:
-        e = normwise(x, y)
-        assert e.max() <= TOL, f"{kind} {alpha} {name}: max {e.max():.2e} at {int(np.argmax(e))}, p99.9 {np.quantile(e, 0.999):.2e}"
+    for name, x, y, z in (("F", F, Fo, rr[1]), ("J", J, Jo, rr[2]), ("H", H, Ho, rr[3]))[:order + 1]:
+        check_spread(f"{kind} {alpha} o{order} {name}", x, y, z)
 
 
 def test_golden_vectors():
@@ -150,11 +176,23 @@ def test_h_symmetry_cd_ef():
 
 
 @pytest.mark.parametrize("n", [1, 2, 27, 63, 64, 65, 1000])
-def test_batch_sizes(oracle, n):
+def test_batch_sizes(oracle, ref, n):
+    """partial wavefronts and lane-group passes; nothing written past the batch"""
+    import torch
+    import matfun_amd
     from tests.gpu_util import call
     G = W.general(n, seed=17 + n)
     rc, F, J, H = call("general", "sqrt", 0.0, G, 2)
     _, Fo, Jo, Ho = oracle.gmf("sqrt", 0.0, G, 2)
+    rr = ref.gmf("sqrt", 0.0, G, 2) if ref is not None else (0, None, None, None)
     assert rc == 0
-    for x, y in ((F, Fo), (J, Jo), (H, Ho)):
-        assert normwise(x, y).max() <= TOL
+    for name, x, y, z in (("F", F, Fo, rr[1]), ("J", J, Jo, rr[2]), ("H", H, Ho, rr[3])):
+        check_spread(name, x, y, z)
+    lib = matfun_amd.library()
+    dA = torch.from_numpy(G).cuda()
+    outs = [torch.full((n + 64, k), 7.0, dtype=torch.float64, device="cuda") for k in (9, 81, 729)]
+    assert lib.gmf_hip_eval(3, 0.0, n, dA.data_ptr(), *[o.data_ptr() for o in outs],
+                            torch.cuda.current_stream().cuda_stream) == 0
+    torch.cuda.synchronize()
+    for o in outs:
+        assert (o[n:] == 7.0).all()

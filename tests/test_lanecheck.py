@@ -22,7 +22,7 @@ KC = {"exp": 0, "log": 1, "pow": 2, "sqrt": 3}
 @pytest.fixture(scope="module")
 def lane():
     lib = ctypes.CDLL(LIB)
-    for f in (lib.lanecheck_smf, lib.lanecheck_gmf):
+    for f in (lib.lanecheck_smf, lib.lanecheck_smf_matrix, lib.lanecheck_gmf):
         f.argtypes = [ctypes.c_int, ctypes.c_double, ctypes.c_int64] + [ctypes.c_void_p] * 4
     return lib
 
@@ -48,6 +48,20 @@ def test_smf_lane_math_vs_oracle(lane, kind, alpha):
         for name, x, y in zip("FJH", got[:order + 1], ref[:order + 1]):
             e = normwise(x, y)
             assert e.max() <= 1e-12, (kind, alpha, order, name, e.max(), int(np.argmax(e)))
+
+
+@pytest.mark.parametrize("kind,alpha", KINDS)
+def test_smf_matrix_layout_maps_vs_oracle(lane, kind, alpha):
+    """the kernels' output index maps (distinct values -> matrix layout with weights 1, 1/2, 1/4)"""
+    o = Checker("oracle")
+    A = W.spd(500, seed=33)
+    A[7, 1] += 0.1                                   # not symmetric: NaN results
+    got = run(lane.lanecheck_smf_matrix, kind, alpha, A, 9, 2)
+    _, *ref = o.smf_matrix(kind, alpha, A, 2)
+    for name, x, y in zip("FJH", got, ref):
+        e = normwise(x, y)
+        assert e.max() <= 1e-12, (kind, alpha, name, e.max(), int(np.argmax(e)))
+        assert np.isnan(x[7]).all()
 
 
 @pytest.mark.parametrize("kind,alpha", KINDS)
