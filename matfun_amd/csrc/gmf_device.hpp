@@ -426,30 +426,35 @@ MF_DEV void gmf_lane(const double* a, double al, const double* __restrict__ bt, 
       E[c] = inG[dd_added(c)] ? mk(0.0, 0.0) : (par < 0 ? mk(1.0, 0.0) : E[par]);
       acc[c] = mk(0.0, 0.0);
     }
-    // the selector of the pole factor per entry: g = q[multiplicity of the outside node]
-    int gsel[ND];
-#pragma unroll
-    for (int c = 0; c < ND; c++) gsel[c] = pk[0] ? dd_mult(c, 0) : (pk[1] ? dd_mult(c, 1) : (pk[2] ? dd_mult(c, 2) : 0));
     double q1 = 0.0, q2 = 0.0, q3 = 0.0;
+    double bnext = 0.0;                         // (b_j loaded one iteration ahead of its use)
     for (int j = 0; j < K; j++) {
-      const double bj = j == 0 ? b0 : bt[j];
+      const double bj = j == 0 ? b0 : bnext;
+      bnext = bt[j + 1 < GMF_KMAX ? j + 1 : GMF_KMAX - 1];
       const double g1 = (bj - q1) * iap, g2 = (g1 - q2) * iap, g3 = (g2 - q3) * iap;
       q1 = g1; q2 = g2; q3 = g3;
-      // children after parents; E[c] is the value at j - 1 until overwritten; a child whose node is outside the
-      // group copies its parent's new value, one inside adds u times its own old value to the parent's old value
-      cpx En[ND];
+      // E in place: children after their parents in the list, so a descending pass sees the parents' values at
+      // j - 1 (a child whose node is in the group: parent(j-1) + u E(j-1)), and an ascending pass then copies the
+      // parent's new value into a child whose node is outside the group.  (The descending pass updates every entry:
+      // an entry outside the group is read by its children before it is updated, and overwritten by the copy.)
+#pragma unroll
+      for (int c = ND - 1; c >= 0; c--) {
+        const int par = dd_parent(c), i = dd_added(c);
+        const cpx epp = par < 0 ? mk(j == 0 ? 1.0 : 0.0, 0.0) : E[par];
+        E[c] = add(epp, mul(u[i], E[c]));
+      }
 #pragma unroll
       for (int c = 0; c < ND; c++) {
         const int par = dd_parent(c), i = dd_added(c);
-        const cpx epp = par < 0 ? mk(j == 0 ? 1.0 : 0.0, 0.0) : E[par];
-        const cpx epc = par < 0 ? mk(0.0, 0.0) : En[par];
-        En[c] = inG[i] ? add(epp, mul(u[i], E[c])) : epc;
-        const int gs = gsel[c];
-        const double g = gs == 0 ? bj : (gs == 1 ? g1 : (gs == 2 ? g2 : g3));
-        acc[c] = add(acc[c], scale(En[c], g));
+        E[c] = pick(inG[i], E[c], par < 0 ? mk(0.0, 0.0) : E[par]);
+        // pole factor: the multiplicity of the node outside a pair group selects G^(r) (masks loop-invariant,
+        // indices constant)
+        const double qa = dd_a(c) == 0 ? bj : (dd_a(c) == 1 ? g1 : (dd_a(c) == 2 ? g2 : g3));
+        const double qb = dd_b(c) == 0 ? bj : (dd_b(c) == 1 ? g1 : (dd_b(c) == 2 ? g2 : g3));
+        const double qc = dd_c(c) == 0 ? bj : (dd_c(c) == 1 ? g1 : (dd_c(c) == 2 ? g2 : g3));
+        const double g = pk[0] ? qa : (pk[1] ? qb : (pk[2] ? qc : bj));
+        acc[c] = add(acc[c], scale(E[c], g));
       }
-#pragma unroll
-      for (int c = 0; c < ND; c++) E[c] = En[c];
     }
 #pragma unroll
     for (int c = 0; c < ND; c++) {
@@ -596,17 +601,36 @@ MF_DEV void gmf_lane(const double* a, double al, const double* __restrict__ bt, 
   }
 }
 
-// ---- assembly of J and H from the staged per-matrix results (r_m[k] at r[9 m + k]); one lane of a 27-lane group
-// J_abcd = sc1 sum_m r_m[a][c] Y_m[d][b], Y_m = sum_n D2_mn r_n  (gmf_kernel.hpp:636-647); lane (b, c, d), a = 0..2
+// ---- assembly of J and H from the staged per-matrix results of one matrix: r1, r2 (9 each, row-major; r_0 = I is
+// implied), the distinct entries of the symmetric D2 (6, packed by sym3) and D3 (10, packed by sym33), and the scale
+// factors.  D2 and D3 are symmetric: T acts alike on every index of the symmetric divided-difference tensors.
+MF_DEV constexpr int sym3(int a, int b) { return a == b ? a : a + b + 2; }                  // (0,1) 3, (0,2) 4, (1,2) 5
+MF_DEV constexpr int sym33s(int i, int j, int k) {                                          // i <= j <= k
+  return i == 0 ? (j == 0 ? k : (j == 1 ? 2 + k : 5)) : (i == 1 ? (j == 1 ? 5 + k : 8) : 9);
+}
+MF_DEV constexpr int sym33(int i, int j, int k) {
+  return sym33s(i < j ? (i < k ? i : k) : (j < k ? j : k),
+                i + j + k - (i < j ? (i < k ? i : k) : (j < k ? j : k)) - (i > j ? (i > k ? i : k) : (j > k ? j : k)),
+                i > j ? (i > k ? i : k) : (j > k ? j : k));
+}
+
+// element (row, col) of r_m (row, col may differ between lanes: r_0 = I by comparison)
 template <class T>
-MF_DEV void gmf_j_lane(const T* r, const T* D2, T sc1, int b, int c, int d, T* out) {
-  T Y[3];
+MF_DEV T rget(const T* r1, const T* r2, int m, int row, int col) {
+  return m == 0 ? (row == col ? (T)1 : (T)0) : (m == 1 ? r1[3 * row + col] : r2[3 * row + col]);
+}
+
+// J_abcd = sc1 sum_mn D2_mn r_m[a][c] r_n[d][b]  (gmf_kernel.hpp:636-647); lane (b, c, d): out[a], a = 0..2
+template <class T>
+MF_DEV void gmf_j_lane(const T* r1, const T* r2, const T* d2s, T sc1, int b, int c, int d, T* out) {
+  T rdb[3], Y[3];
 #pragma unroll
-  for (int mm = 0; mm < 3; mm++)
-    Y[mm] = D2[3 * mm] * r[3 * d + b] + D2[3 * mm + 1] * r[9 + 3 * d + b] + D2[3 * mm + 2] * r[18 + 3 * d + b];
+  for (int n = 0; n < 3; n++) rdb[n] = rget(r1, r2, n, d, b);
+#pragma unroll
+  for (int mm = 0; mm < 3; mm++) Y[mm] = d2s[sym3(mm, 0)] * rdb[0] + d2s[sym3(mm, 1)] * rdb[1] + d2s[sym3(mm, 2)] * rdb[2];
 #pragma unroll
   for (int aa = 0; aa < 3; aa++)
-    out[aa] = (r[3 * aa + c] * Y[0] + r[9 + 3 * aa + c] * Y[1] + r[18 + 3 * aa + c] * Y[2]) * sc1;
+    out[aa] = (rget(r1, r2, 0, aa, c) * Y[0] + rget(r1, r2, 1, aa, c) * Y[1] + rget(r1, r2, 2, aa, c) * Y[2]) * sc1;
 }
 
 // H_abcdef = sc2 (T6_acdefb + T6_aefcdb), T6_acdefb = sum_mnp D3_mnp r_m[a][c] r_n[d][e] r_p[f][b]
@@ -614,7 +638,17 @@ MF_DEV void gmf_j_lane(const T* r, const T* D2, T sc1, int b, int c, int d, T* o
 // Yh[m][(de)(fb)] = sum_n r_n[d][e] Z[mn][fb], H = sc2 (sum_m r_m[a][c] X1[m][b] + sum_m r_m[a][e] X2[m][c][b]),
 // X1[m][b] = Yh[m][(de)(fb)], X2[m][c][b] = Yh[m][(fc)(db)]; out[9a + 3b + c]
 template <class T>
-MF_DEV void gmf_h_lane(const T* r, const T* D3, T sc2, int d, int e, int f, T* out) {
+MF_DEV void gmf_h_lane(const T* r1, const T* r2, const T* d3s, T sc2, int d, int e, int f, T* out) {
+  T rfb[3][3], rdb[3][3], rde[3];   // r_p[f][b], r_p[d][b], r_p[d][e]
+#pragma unroll
+  for (int p = 0; p < 3; p++) {
+    rde[p] = rget(r1, r2, p, d, e);
+#pragma unroll
+    for (int b = 0; b < 3; b++) {
+      rfb[p][b] = rget(r1, r2, p, f, b);
+      rdb[p][b] = rget(r1, r2, p, d, b);
+    }
+  }
   T X1[3][3], X2[3][3][3];
 #pragma unroll
   for (int mm = 0; mm < 3; mm++) {
@@ -623,27 +657,29 @@ MF_DEV void gmf_h_lane(const T* r, const T* D3, T sc2, int d, int e, int f, T* o
     for (int nn = 0; nn < 3; nn++)
 #pragma unroll
       for (int b = 0; b < 3; b++) {
-        const T* D = D3 + 9 * mm + 3 * nn;
-        Z1[nn][b] = D[0] * r[3 * f + b] + D[1] * r[9 + 3 * f + b] + D[2] * r[18 + 3 * f + b];
-        Z2[nn][b] = D[0] * r[3 * d + b] + D[1] * r[9 + 3 * d + b] + D[2] * r[18 + 3 * d + b];
+        const T D0 = d3s[sym33(mm, nn, 0)], D1 = d3s[sym33(mm, nn, 1)], D2 = d3s[sym33(mm, nn, 2)];
+        Z1[nn][b] = D0 * rfb[0][b] + D1 * rfb[1][b] + D2 * rfb[2][b];
+        Z2[nn][b] = D0 * rdb[0][b] + D1 * rdb[1][b] + D2 * rdb[2][b];
       }
 #pragma unroll
     for (int b = 0; b < 3; b++) {
-      X1[mm][b] = r[3 * d + e] * Z1[0][b] + r[9 + 3 * d + e] * Z1[1][b] + r[18 + 3 * d + e] * Z1[2][b];
+      X1[mm][b] = rde[0] * Z1[0][b] + rde[1] * Z1[1][b] + rde[2] * Z1[2][b];
 #pragma unroll
       for (int c = 0; c < 3; c++)
-        X2[mm][c][b] = r[3 * f + c] * Z2[0][b] + r[9 + 3 * f + c] * Z2[1][b] + r[18 + 3 * f + c] * Z2[2][b];
+        X2[mm][c][b] = rfb[0][c] * Z2[0][b] + rfb[1][c] * Z2[1][b] + rfb[2][c] * Z2[2][b];
     }
   }
 #pragma unroll
   for (int aa = 0; aa < 3; aa++) {
-    const T re0 = r[3 * aa + e], re1 = r[9 + 3 * aa + e], re2 = r[18 + 3 * aa + e];
+    const T re0 = rget(r1, r2, 0, aa, e), re1 = rget(r1, r2, 1, aa, e), re2 = rget(r1, r2, 2, aa, e);
 #pragma unroll
-    for (int b = 0; b < 3; b++)
+    for (int c = 0; c < 3; c++) {
+      const T rc0 = rget(r1, r2, 0, aa, c), rc1 = rget(r1, r2, 1, aa, c), rc2 = rget(r1, r2, 2, aa, c);
 #pragma unroll
-      for (int c = 0; c < 3; c++)
-        out[9 * aa + 3 * b + c] = (r[3 * aa + c] * X1[0][b] + r[9 + 3 * aa + c] * X1[1][b] + r[18 + 3 * aa + c] * X1[2][b] +
-                                   re0 * X2[0][c][b] + re1 * X2[1][c][b] + re2 * X2[2][c][b]) * sc2;
+      for (int b = 0; b < 3; b++)
+        out[9 * aa + 3 * b + c] = (rc0 * X1[0][b] + rc1 * X1[1][b] + rc2 * X1[2][b] + re0 * X2[0][c][b] +
+                                   re1 * X2[1][c][b] + re2 * X2[2][c][b]) * sc2;
+    }
   }
 }
 

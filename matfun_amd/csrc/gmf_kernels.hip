@@ -14,14 +14,14 @@ namespace mf {
 constexpr int GWAVE = 64;
 
 // LDS layout (elements of OT):
-//   main  [64][MS]      r (27), then D3 (27) and sc2 at 54 (order 2)
-//   x     [XS]          [0, 1458): the J (2 x 81) / H (2 x 729) output tile of a pass;
-//                       before the H passes also [XD2, +640): D2 (9) and sc1 per matrix, [XF, +576): F per matrix
+//   main [64][MS]   r1 (9), r2 (9); order 2: the 10 distinct D3 and sc2 (MS = 29; order 1: 19)
+//   x    [XS]       phase 1: F of the 64 matrices [0, 576), then D2 (6 distinct) and sc1 per matrix [162, 610);
+//                   J passes: the 2 x 81 output tile [0, 162); H passes: a half-block tile 2 x 378 (or 2 x 351)
 template <int ORDER>
 struct GmfLds {
-  static constexpr int MS = ORDER >= 2 ? 55 : 27;
-  static constexpr int XD2 = 162, XF = ORDER == 0 ? 0 : XD2 + 640;
-  static constexpr int XS = ORDER == 0 ? 576 : (ORDER == 1 ? XF + 576 : 1458);
+  static constexpr int MS = ORDER >= 2 ? 29 : 19;
+  static constexpr int XAUX = 162;
+  static constexpr int XS = ORDER == 0 ? 576 : (ORDER == 1 ? 610 : 756);
   static constexpr int MAIN = ORDER == 0 ? 0 : 64 * MS;
 };
 
@@ -47,39 +47,46 @@ __global__ void __launch_bounds__(GWAVE) gmf_kernel(int64_t n, const double* __r
   __syncthreads();
 
   // ---- phase 1
-  {
-    GmfOut o;
-    gmf_lane<KIND, ORDER>(a, al, bt.b, o);
+  GmfOut o;
+  gmf_lane<KIND, ORDER>(a, al, bt.b, o);
+  if (ORDER >= 1) {
+    OT* my = mainr + lane * L::MS;
 #pragma unroll
-    for (int k = 0; k < 9; k++) xr[L::XF + lane * 9 + k] = (OT)o.F[k];
-    if (ORDER >= 1) {
-      OT* my = mainr + lane * L::MS;
-#pragma unroll
-      for (int k = 0; k < 27; k++) my[k] = (OT)o.r[k / 9][k % 9];
-#pragma unroll
-      for (int k = 0; k < 9; k++) xr[L::XD2 + lane * 10 + k] = (OT)o.D2[k];
-      xr[L::XD2 + lane * 10 + 9] = (OT)o.sc1;
-    }
+    for (int k = 0; k < 9; k++) { my[k] = (OT)o.r[1][k]; my[9 + k] = (OT)o.r[2][k]; }
     if (ORDER >= 2) {
-      OT* my = mainr + lane * L::MS;
 #pragma unroll
-      for (int k = 0; k < 27; k++) my[27 + k] = (OT)o.D3[k];
-      my[54] = (OT)o.sc2;
+      for (int i = 0; i < 3; i++)
+#pragma unroll
+        for (int j = i; j < 3; j++)
+#pragma unroll
+          for (int k = j; k < 3; k++) my[18 + sym33s(i, j, k)] = (OT)o.D3[9 * i + 3 * j + k];
+      my[28] = (OT)o.sc2;
     }
   }
+#pragma unroll
+  for (int k = 0; k < 9; k++) xr[lane * 9 + k] = (OT)o.F[k];
   __syncthreads();
-  for (int t = lane; t < 9 * nm; t += GWAVE) F[base * 9 + t] = xr[L::XF + t];
+  for (int t = lane; t < 9 * nm; t += GWAVE) F[base * 9 + t] = xr[t];
   if (ORDER < 1) return;
+  __syncthreads();
+#pragma unroll
+  for (int i = 0; i < 3; i++)
+#pragma unroll
+    for (int j = i; j < 3; j++) xr[L::XAUX + lane * 7 + sym3(i, j)] = (OT)o.D2[3 * i + j];
+  xr[L::XAUX + lane * 7 + 6] = (OT)o.sc1;
+  __syncthreads();
 
-  // ---- phase 2: lane (g, L) = matrix g of the pass, position L = 9 u + 3 v + w of a 27-lane group
+  // ---- phase 2: lane (g, Lg) = matrix g of the pass, position Lg = 9 u + 3 v + w of a 27-lane group
   constexpr int G = 27, MPP = GWAVE / G;
   const int g = lane / G, Lg = lane - g * G;
   for (int p0 = 0; p0 < nm; p0 += MPP) {          // J: J_abcd for a = 0..2 at 27 a + (9b + 3c + d)
     const int cnt = nm - p0 < MPP ? nm - p0 : MPP;
     if (g < cnt) {
       const int j = p0 + g;
+      const OT* s = mainr + j * L::MS;
+      const OT* aux = xr + L::XAUX + j * 7;
       OT out[3];
-      gmf_j_lane(mainr + j * L::MS, xr + L::XD2 + j * 10, xr[L::XD2 + j * 10 + 9], Lg / 9, (Lg / 3) % 3, Lg % 3, out);
+      gmf_j_lane(s, s + 9, aux, aux[6], Lg / 9, (Lg / 3) % 3, Lg % 3, out);
 #pragma unroll
       for (int k = 0; k < 3; k++) xr[g * 81 + 27 * k + Lg] = out[k];
     }
@@ -90,16 +97,26 @@ __global__ void __launch_bounds__(GWAVE) gmf_kernel(int64_t n, const double* __r
   if (ORDER < 2) return;
   for (int p0 = 0; p0 < nm; p0 += MPP) {          // H: H_abcdef for (a,b,c) = 0..26 at 27 (9a + 3b + c) + (9d + 3e + f)
     const int cnt = nm - p0 < MPP ? nm - p0 : MPP;
+    OT out[27];
     if (g < cnt) {
       const OT* s = mainr + (p0 + g) * L::MS;
-      OT out[27];
-      gmf_h_lane(s, s + 27, s[54], Lg / 9, (Lg / 3) % 3, Lg % 3, out);
-#pragma unroll
-      for (int k = 0; k < 27; k++) xr[g * 729 + 27 * k + Lg] = out[k];
+      gmf_h_lane(s, s + 9, s + 18, s[28], Lg / 9, (Lg / 3) % 3, Lg % 3, out);
     }
-    __syncthreads();
-    for (int t = lane; t < 729 * cnt; t += GWAVE) H[(base + p0) * 729 + t] = xr[t];
-    __syncthreads();
+#pragma unroll
+    for (int half = 0; half < 2; half++) {        // rows k < 14 (378 elements), then k >= 14 (351)
+      constexpr int K0[2] = {0, 14}, SZ[2] = {378, 351};
+      if (g < cnt) {
+#pragma unroll
+        for (int k = 0; k < 14; k++)
+          if (K0[half] + k < 27) xr[g * SZ[half] + 27 * k + Lg] = out[K0[half] + k];
+      }
+      __syncthreads();
+      for (int t = lane; t < SZ[half] * cnt; t += GWAVE) {
+        const int gg = t >= SZ[half] ? 1 : 0, e = t - gg * SZ[half];
+        H[(base + p0 + gg) * 729 + 27 * K0[half] + e] = xr[t];
+      }
+      __syncthreads();
+    }
   }
 }
 

@@ -19,7 +19,32 @@ constexpr int SB = 19;             // V (9) + D2 (10)
 constexpr int HPP = 10;            // matrices per phase-B pass (6 lanes each)
 constexpr int SH = 127;            // staged H values per matrix (6 rows x 21, +1: odd stride)
 
-template <int KIND, bool MAT, int ORDER, class OT>
+// out[t] = get(j, e) for t = j N + e < count, by the wave: with PAIRS two consecutive elements per lane and one
+// vector store (16 B FP64 / 8 B FP32; out must be aligned to it), else one element per lane
+template <int N, bool PAIRS, class OT, class Get>
+MF_DEV void copy_out(OT* __restrict__ out, int count, int lane, Get get) {
+  if (PAIRS) {
+    typedef OT v2 __attribute__((ext_vector_type(2)));
+    int t = 2 * lane, j = t / N, e = t - j * N;
+    for (; t + 1 < count; t += 2 * WAVE) {
+      const int e1 = e + 1 == N ? 0 : e + 1, j1 = e + 1 == N ? j + 1 : j;
+      const v2 val = {get(j, e), get(j1, e1)};
+      *reinterpret_cast<v2*>(out + t) = val;
+      e += 2 * WAVE;
+      while (e >= N) { e -= N; j++; }
+    }
+    if (t < count) out[t] = get(j, e);         // (odd count: the last element)
+  } else {
+    int j = lane / N, e = lane - j * N;
+    for (int t = lane; t < count; t += WAVE) {
+      out[t] = get(j, e);
+      e += WAVE;
+      while (e >= N) { e -= N; j++; }
+    }
+  }
+}
+
+template <int KIND, bool MAT, int ORDER, class OT, bool PAIRS>
 __global__ void __launch_bounds__(WAVE) smf_kernel(int64_t n, const double* __restrict__ x, OT* __restrict__ F,
                                                    OT* __restrict__ J, OT* __restrict__ H, double al, SmfCoef c2) {
   constexpr int NIN = MAT ? 9 : 6, NF = MAT ? 9 : 6, NJ = NF * NF, NH = NJ * NF;
@@ -90,17 +115,11 @@ __global__ void __launch_bounds__(WAVE) smf_kernel(int64_t n, const double* __re
     F[base * NF + t] = t1[j * SA + f_src(MAT, e)];
   }
   if (ORDER < 1) return;
-  {
-    int j = lane / NJ, e = lane - j * NJ;     // t = j NJ + e, advanced by the wave width (NJ >= 36: <= 2 wraps)
-    OT* out = J + base * NJ;
-    for (int t = lane; t < NJ * nm; t += WAVE) {
-      const int u = lutJ[e];
-      const OT val = t1[j * SA + 6 + (u & 255)];
-      out[t] = (u >> 8) ? (OT)2 * val : val;
-      e += WAVE;
-      while (e >= NJ) { e -= NJ; j++; }
-    }
-  }
+  copy_out<NJ, PAIRS>(J + base * NJ, NJ * nm, lane, [&](int j, int e) {
+    const int u = lutJ[e];
+    const OT val = t1[j * SA + 6 + (u & 255)];
+    return (u >> 8) ? (OT)2 * val : val;
+  });
   if (ORDER < 2) return;
 
   // ---- phase B: lane = (matrix jj of the pass, Voigt row i)
@@ -117,24 +136,32 @@ __global__ void __launch_bounds__(WAVE) smf_kernel(int64_t n, const double* __re
       for (int k = 0; k < 21; k++) t1[jj * SH + ir * 21 + k] = h[k];
     }
     __syncthreads();
-    int j = lane / NH, e = lane - j * NH;     // (NH >= 216 > 64)
-    OT* out = H + (base + p0) * NH;
-    for (int t = lane; t < NH * cnt; t += WAVE) {
+    copy_out<NH, PAIRS>(H + (base + p0) * NH, NH * cnt, lane, [&](int j, int e) {
       const int u = lutH[e];
       const OT val = t1[j * SH + (u & 255)];
-      out[t] = (u >> 8) == 0 ? val : ((u >> 8) == 1 ? (OT)0.5 * val : (OT)0.25 * val);
-      e += WAVE;
-      if (e >= NH) { e -= NH; j++; }   // (NH > 64: at most one wrap)
-    }
+      return (u >> 8) == 0 ? val : ((u >> 8) == 1 ? (OT)0.5 * val : (OT)0.25 * val);
+    });
   }
 }
 
+template <int KIND, bool MAT, class OT, bool PAIRS>
+static void launch_order(hipStream_t st, unsigned blocks, int64_t n, const double* x, OT* F, OT* J, OT* H, double al,
+                         const SmfCoef& c2) {
+  if (H) hipLaunchKernelGGL((smf_kernel<KIND, MAT, 2, OT, PAIRS>), dim3(blocks), dim3(WAVE), 0, st, n, x, F, J, H, al, c2);
+  else if (J) hipLaunchKernelGGL((smf_kernel<KIND, MAT, 1, OT, PAIRS>), dim3(blocks), dim3(WAVE), 0, st, n, x, F, J, H, al, c2);
+  else hipLaunchKernelGGL((smf_kernel<KIND, MAT, 0, OT, false>), dim3(blocks), dim3(WAVE), 0, st, n, x, F, J, H, al, c2);
+}
+
+// paired stores need J and H aligned to two elements (the per-wave regions then stay aligned: 64 and 10 matrices of
+// 81 / 729 (36 / 216) elements are even counts)
 template <int KIND, bool MAT, class OT>
 static void launch_kind(hipStream_t st, unsigned blocks, int64_t n, const double* x, OT* F, OT* J, OT* H, double al,
                         const SmfCoef& c2) {
-  if (H) hipLaunchKernelGGL((smf_kernel<KIND, MAT, 2, OT>), dim3(blocks), dim3(WAVE), 0, st, n, x, F, J, H, al, c2);
-  else if (J) hipLaunchKernelGGL((smf_kernel<KIND, MAT, 1, OT>), dim3(blocks), dim3(WAVE), 0, st, n, x, F, J, H, al, c2);
-  else hipLaunchKernelGGL((smf_kernel<KIND, MAT, 0, OT>), dim3(blocks), dim3(WAVE), 0, st, n, x, F, J, H, al, c2);
+  const uintptr_t al2 = 2 * sizeof(OT) - 1;
+  if ((!J || ((uintptr_t)J & al2) == 0) && (!H || ((uintptr_t)H & al2) == 0))
+    launch_order<KIND, MAT, OT, true>(st, blocks, n, x, F, J, H, al, c2);
+  else
+    launch_order<KIND, MAT, OT, false>(st, blocks, n, x, F, J, H, al, c2);
 }
 
 template <class OT>

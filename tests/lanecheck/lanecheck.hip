@@ -52,15 +52,21 @@ void gmf_all(double al, int64_t n, const double* A, double* F, double* J, double
     mf::gmf_lane<KIND, ORDER>(A + 9 * m, al, bt.b, o);
     for (int k = 0; k < 9; k++) F[9 * m + k] = o.F[k];
     if (ORDER < 1) continue;
-    double r[27], out[27];
-    for (int k = 0; k < 27; k++) r[k] = o.r[k / 9][k % 9];
+    // the values the kernel stages: r1, r2, the distinct entries of the symmetric D2 / D3
+    double r1[9], r2[9], d2s[6], d3s[10], out[27];
+    for (int k = 0; k < 9; k++) { r1[k] = o.r[1][k]; r2[k] = o.r[2][k]; }
+    for (int i = 0; i < 3; i++)
+      for (int j = i; j < 3; j++) d2s[mf::sym3(i, j)] = o.D2[3 * i + j];
     for (int L = 0; L < 27; L++) {
-      mf::gmf_j_lane(r, o.D2, o.sc1, L / 9, (L / 3) % 3, L % 3, out);
+      mf::gmf_j_lane(r1, r2, d2s, o.sc1, L / 9, (L / 3) % 3, L % 3, out);
       for (int a = 0; a < 3; a++) J[81 * m + 27 * a + L] = out[a];
     }
     if (ORDER < 2) continue;
+    for (int i = 0; i < 3; i++)
+      for (int j = i; j < 3; j++)
+        for (int k = j; k < 3; k++) d3s[mf::sym33s(i, j, k)] = o.D3[9 * i + 3 * j + k];
     for (int L = 0; L < 27; L++) {
-      mf::gmf_h_lane(r, o.D3, o.sc2, L / 9, (L / 3) % 3, L % 3, out);
+      mf::gmf_h_lane(r1, r2, d3s, o.sc2, L / 9, (L / 3) % 3, L % 3, out);
       for (int k = 0; k < 27; k++) H[729 * m + 27 * k + L] = out[k];
     }
   }

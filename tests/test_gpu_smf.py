@@ -203,3 +203,26 @@ def test_extreme_scales():
         if k < 2:
             e = max(e, maxrel(H, H0 / (s * np.sqrt(s))))
         assert e <= 1e-14, (s, e)
+
+
+@pytest.mark.parametrize("mixed", [False, True])
+def test_unaligned_output_pointers(oracle, mixed):
+    """outputs that are not aligned to two elements take the one-element store path: same results"""
+    import torch
+    import matfun_amd
+    lib = matfun_amd.library()
+    n = 130
+    A = W.spd(n, seed=77)
+    dt = torch.float32 if mixed else torch.float64
+    fn = lib.smf_hip_eval_matrix_mixed if mixed else lib.smf_hip_eval_matrix
+    dA = torch.from_numpy(A).cuda()
+    bufs = [torch.full((n * k + 1,), 7.0, dtype=dt, device="cuda") for k in (9, 81, 729)]
+    views = [b[1:] for b in bufs]                   # one element past an aligned allocation
+    assert fn(1, 0.0, n, dA.data_ptr(), *[v.data_ptr() for v in views], torch.cuda.current_stream().cuda_stream) == 0
+    torch.cuda.synchronize()
+    _, Fo, Jo, Ho = oracle.smf_matrix("log", 0.0, A, 2)
+    tol = 1e-4 if mixed else 1e-12
+    for v, ref in zip(views, (Fo, Jo, Ho)):
+        assert normwise(v.double().cpu().numpy().reshape(n, -1), ref).max() <= tol
+    for b in bufs:
+        assert b[0].item() == 7.0
