@@ -41,6 +41,19 @@ def bytes_per_matrix(order, out_bytes):
     return 72 + out_bytes * (9 + (81 if order >= 1 else 0) + (729 if order >= 2 else 0))
 
 
+def profiled_traffic(n):
+    """HBM bytes per dispatch of the headline kernel from the committed rocprofv3 FETCH_SIZE / WRITE_SIZE summary
+    (profiles/smf_order2_matrix_fp64_traffic.md, counters scaled by a 2 GiB device copy), scaled to n matrices"""
+    path = os.path.join(ROOT, "profiles", "smf_order2_matrix_fp64_traffic.md")
+    try:
+        for line in open(path):
+            if line.startswith("traffic_bytes_per_dispatch_n4194304:"):
+                return float(line.split(":")[1]) * n / 4194304
+    except OSError:
+        pass
+    return None
+
+
 def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
@@ -200,7 +213,7 @@ def main():
                       "n_per_function": n, "functions": ["exp", "log", "sqrt", "pow0.3"], "order": 2,
                       "layout": "matrix", "bytes_per_matrix": head["bytes_per_matrix"]},
            "roofline": {"bound": "hbm", "achieved": head["achieved_GBs"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                        "frac": head["frac"], "traffic": None, "kernel": "mf::smf_kernel<KIND,true,2,double>",
+                        "frac": head["frac"], "traffic": profiled_traffic(n), "kernel": "mf::smf_kernel<KIND,true,2,double>",
                         "avg_kernel_ms": head["avg_kernel_ms"], "measured_copy_GBs": copy_gbs},
            "legs": legs}
     if not args.no_cpu:
