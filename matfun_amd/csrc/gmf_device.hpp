@@ -337,7 +337,8 @@ MF_DEV void gmf_lane(const double* a, double al, const double* __restrict__ bt, 
   }
 
   // ---- groups: all three, else the closest real / conjugate pair with the pole of the third node
-  const double rho_f = (KIND == POW && 3.0 / al < RHO) ? 3.0 / al : RHO;
+  // (|al|: per-matrix exponents may be <= 0 -- an extension; for al > 0 the reference's radius)
+  const double rho_f = (KIND == POW && 3.0 / fabs(al) < RHO) ? 3.0 / fabs(al) : RHO;
   auto group_ok = [&](double c, double R) -> bool {
     if (KIND == EXP) return R <= R_EXP;
     return (m + c > 0) && (R <= rho_f * (m + c));

@@ -103,7 +103,8 @@ MF_DEV double dd2(double li, double lj, double lk, double fi, double fj, double 
     w = hi - mid;
     scl = fm;
   } else {
-    const double tau = (KIND == POW && 3.0 / al < 0.25) ? 3.0 / al : 0.25;
+    // (|al|: per-matrix exponents may be <= 0 -- an extension; for al > 0 the reference's radius)
+    const double tau = (KIND == POW && 3.0 / fabs(al) < 0.25) ? 3.0 / fabs(al) : 0.25;
     cluster = (mid > 0) && (mid - lo <= tau * mid) && (hi - mid <= tau * mid);
     const double ms = cluster ? mid : 1.0;
     u = (lo - mid) / ms;

@@ -332,7 +332,8 @@ static int gmf_core(int kind, int order, double al, const double *a, double *F, 
 #undef NODE_KEY
 
   /* groups: all three nodes, else the closest real or conjugate pair (with the pole of the third node) */
-  const double rho_f = (kind == K_POW && 3.0 / al < RHO) ? 3.0 / al : RHO;
+  /* (|al|: the per-matrix exponent extension also takes al <= 0; for al > 0 this is gmf_kernel.hpp:413) */
+  const double rho_f = (kind == K_POW && 3.0 / fabs(al) < RHO) ? 3.0 / fabs(al) : RHO;
 #define GROUP_OK(c, R) (kind == K_EXP ? ((R) <= R_EXP) : ((m + (c) > 0) && ((R) <= rho_f * (m + (c)))))
   const double cs = (x[0].re + x[1].re + x[2].re) / 3.0;
   const double R3 = dmax(dmax(cp_abs(cp(x[0].re - cs, x[0].im)), cp_abs(cp(x[1].re - cs, x[1].im))),

@@ -59,7 +59,7 @@ int oracle_smf_eval_matrix_alpha(int kind, const double *alpha, int64_t n, const
                                  double *H) {
   if (kind != K_POW || n < 0 || (H && !J)) return -1;
   for (int64_t i = 0; i < n; i++)
-    if (!(alpha[i] > 0)) return -1;
+    if (!isfinite(alpha[i])) return -1;
   smf_matrix_batch(kind, alpha, 0.0, n, A, F, J, H);
   return 0;
 }
@@ -93,7 +93,7 @@ int oracle_gmf_eval(int kind, double alpha, int64_t n, const double *A, double *
 int oracle_gmf_eval_alpha(int kind, const double *alpha, int64_t n, const double *A, double *F, double *J, double *H) {
   if (kind != K_POW || n < 0 || (H && !J)) return -1;
   for (int64_t i = 0; i < n; i++)
-    if (!(alpha[i] > 0)) return -1;
+    if (!isfinite(alpha[i])) return -1;
   gmf_batch(kind, alpha, 0.0, n, A, F, J, H);
   return 0;
 }

@@ -103,7 +103,8 @@ static double dd2(int kind, double li, double lj, double lk, double fi, double f
     w = hi - mid;
     scale = fm;
   } else {
-    const double tau = (kind == K_POW && 3.0 / al < 0.25) ? 3.0 / al : 0.25;
+    /* (|al|: the per-matrix exponent extension also takes al <= 0; for al > 0 this is smf_kernel.hpp:173) */
+    const double tau = (kind == K_POW && 3.0 / fabs(al) < 0.25) ? 3.0 / fabs(al) : 0.25;
     cluster = (mid > 0) && (mid - lo <= tau * mid) && (hi - mid <= tau * mid);
     const double ms = cluster ? mid : 1.0;
     u = (lo - mid) / ms;

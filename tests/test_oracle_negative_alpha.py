@@ -1,0 +1,113 @@
+"""Per-matrix exponents alpha <= 0 (an extension: the reference API requires alpha > 0; configurations with alpha drawn
+from U[-1, 1] need it).  The oracle's direct evaluation at alpha < 0 is pinned through the identity
+A^alpha = (A^-1)^|alpha| evaluated by the oracle at the positive exponent |alpha| (which is bitwise the reference's
+computation) and the chain rule:
+    B = A^-1 (adjugate / det in long double, then rounded),  dB[E] = -B E B,  d2B[E1, E2] = B E1 B E2 B + B E2 B E1 B
+    F(alpha; A) = F(|alpha|; B),  J(alpha; A)[E] = J(|alpha|; B)[dB[E]],
+    H(alpha; A)[E1, E2] = H(|alpha|; B)[dB[E1], dB[E2]] + J(|alpha|; B)[d2B[E1, E2]]
+with the symmetrised directions (E + E^T) / 2 for smf (its derivatives are those of A -> f((A + A^T)/2)).
+Special cases: alpha = -1 gives A^-1; alpha = 0 gives I with J = H = 0.
+Tolerance 1e-10 per matrix (normwise): the two sides are different floating-point computations of the same function
+on matrices with eigenvalues in [0.1, 10] (condition <= 1e2)."""
+import numpy as np
+import pytest
+
+from oracle.pyoracle import Checker
+from matfun_amd import workloads as W
+from tests.parity import normwise
+
+
+def inv_ld(A):
+    """inverses of 3x3 matrices by the adjugate in long double (correctly rounded to a few ulp)"""
+    M = np.asarray(A, dtype=np.longdouble).reshape(-1, 3, 3)
+    C = np.empty_like(M)
+    for i in range(3):
+        for j in range(3):
+            r = [k for k in range(3) if k != i]
+            c = [k for k in range(3) if k != j]
+            C[:, j, i] = (-1) ** (i + j) * (M[:, r[0], c[0]] * M[:, r[1], c[1]] - M[:, r[0], c[1]] * M[:, r[1], c[0]])
+    det = (M[:, 0, 0] * C[:, 0, 0] + M[:, 0, 1] * C[:, 1, 0] + M[:, 0, 2] * C[:, 2, 0])
+    return (C / det[:, None, None]).astype(np.float64).reshape(-1, 9)
+
+
+def directions(sym):
+    E = np.zeros((9, 3, 3))
+    for c in range(9):
+        E[c, c // 3, c % 3] = 1.0
+    return 0.5 * (E + np.swapaxes(E, 1, 2)) if sym else E
+
+
+def chain(B, Fp, Jp, Hp, sym):
+    """F, J, H at alpha from F, J, H at |alpha| evaluated at B = A^-1 (all in the 3x3 matrix layout)"""
+    n = len(B)
+    B = B.reshape(n, 3, 3)
+    E = directions(sym)
+    dB = -np.einsum("nik,ckl,nlj->ncij", B, E, B).reshape(n, 9, 9)            # [n, c, p]
+    BEB = np.einsum("nik,ckl,nlj->ncij", B, E, B)                              # [n, c, 3, 3]
+    # d2B[E_c, E_e] = B E_c B E_e B + B E_e B E_c B
+    BEBE = np.einsum("ncik,ekl,nlj->nceij", BEB, E, B)                          # B E_c B E_e B
+    d2B = (BEBE + np.swapaxes(BEBE, 1, 2)).reshape(n, 9, 9, 9)                  # [n, c, e, p]
+    J = np.einsum("nmp,ncp->nmc", Jp.reshape(n, 9, 9), dB)
+    H = (np.einsum("nmpq,ncp,neq->nmce", Hp.reshape(n, 9, 9, 9), dB, dB) +
+         np.einsum("nmp,ncep->nmce", Jp.reshape(n, 9, 9), d2B))
+    return Fp, J.reshape(n, 81), H.reshape(n, 729)
+
+
+@pytest.fixture(scope="module")
+def oracle():
+    return Checker("oracle")
+
+
+@pytest.mark.parametrize("alpha", [-1.0, -0.75, -0.3, -1e-3])
+def test_smf_negative_alpha_by_inverse(oracle, alpha):
+    A = W.spd(300, seed=41, log10_range=(-1.0, 1.0))
+    n = len(A)
+    _, F, J, H = oracle.smf_matrix_alpha(np.full(n, alpha), A, 2)
+    B = inv_ld(A)
+    B = 0.5 * (B + B.reshape(n, 3, 3).transpose(0, 2, 1).reshape(n, 9))
+    _, Fp, Jp, Hp = oracle.smf_matrix_alpha(np.full(n, -alpha), B, 2)
+    Fr, Jr, Hr = chain(B, Fp, Jp, Hp, sym=True)
+    for name, x, y in (("F", F, Fr), ("J", J, Jr), ("H", H, Hr)):
+        e = normwise(x, y)
+        assert e.max() <= 1e-10, (alpha, name, e.max())
+
+
+@pytest.mark.parametrize("alpha", [-1.0, -0.5, -0.3])
+def test_gmf_negative_alpha_by_inverse(oracle, alpha):
+    G = W.general(400, seed=42)
+    # keep the well-conditioned matrices inside the domain (no real eigenvalue <= 0, moderate condition)
+    ok = np.array([np.linalg.cond(g.reshape(3, 3)) < 50 for g in G])
+    G = G[ok][:150]
+    n = len(G)
+    _, F, J, H = oracle.gmf_alpha(np.full(n, alpha), G, 2)
+    bad = np.isnan(F).any(axis=1)
+    B = inv_ld(G)
+    _, Fp, Jp, Hp = oracle.gmf_alpha(np.full(n, -alpha), B, 2)
+    Fr, Jr, Hr = chain(B, Fp, Jp, Hp, sym=False)
+    for name, x, y in (("F", F, Fr), ("J", J, Jr), ("H", H, Hr)):
+        e = normwise(x[~bad], y[~bad])
+        assert e.max() <= 1e-9, (alpha, name, e.max())
+
+
+def test_alpha_minus_one_is_the_inverse(oracle):
+    A = W.spd(200, seed=43, log10_range=(-1.0, 1.0))
+    _, F, _, _ = oracle.smf_matrix_alpha(np.full(len(A), -1.0), A, 0)
+    assert normwise(F, inv_ld(A)).max() <= 1e-13
+    G = W.general(200, seed=44)
+    _, Fg, _, _ = oracle.gmf_alpha(np.full(len(G), -1.0), G, 0)
+    ok = ~np.isnan(Fg).any(axis=1) & np.array([np.linalg.cond(g.reshape(3, 3)) < 1e3 for g in G])
+    assert normwise(Fg[ok], inv_ld(G)[ok]).max() <= 1e-11
+
+
+def test_alpha_zero_is_identity_with_zero_derivatives(oracle):
+    A = W.spd(100, seed=45)
+    _, F, J, H = oracle.smf_matrix_alpha(np.zeros(len(A)), A, 2)
+    assert np.abs(F - np.eye(3).ravel()).max() <= 1e-14
+    assert np.abs(J).max() == 0.0 and np.abs(H).max() == 0.0
+    G = W.general(100, seed=46)
+    _, Fg, Jg, Hg = oracle.gmf_alpha(np.zeros(len(G)), G, 2)
+    ok = ~np.isnan(Fg).any(axis=1)
+    assert np.abs(Fg[ok] - np.eye(3).ravel()).max() <= 1e-13
+    # gmf assembles J and H from residue sums that cancel to zero only up to rounding (eigenvalue separations of the
+    # random general matrices reach 1e-2, amplifying it by their inverse powers)
+    assert np.abs(Jg[ok]).max() <= 1e-11 and np.abs(Hg[ok]).max() <= 1e-9
