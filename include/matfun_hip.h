@@ -49,6 +49,25 @@ int smf_hip_eval_matrix_mixed(int kind, double alpha, int64_t n, const double *A
                               void *stream);
 int gmf_hip_eval_mixed(int kind, double alpha, int64_t n, const double *A, float *F, float *J, float *H, void *stream);
 
+/* ---- powers with one exponent per matrix (no counterpart in the reference API, which takes one alpha > 0 per call):
+ * F_i = A_i^alpha[i] and its derivatives, for batches that mix exponents (e.g. a material parameter per
+ * integration point).  alpha is a DEVICE array of n doubles; any finite value is accepted: alpha = 0 gives I with
+ * zero derivatives, alpha < 0 the powers of the inverse (the Taylor radii 3/alpha of the reference become 3/|alpha|,
+ * identical for alpha > 0, where the results are those of smf_hip_eval* / gmf_hip_eval with kind 2).  A matrix whose
+ * exponent is not finite gets NaN results.  Same layouts as above; returns 0, -1 (n < 0, H without J, alpha NULL with
+ * n > 0), -2 (launch error / batch too large). */
+int smf_hip_eval_pow(const double *alpha, int64_t n, const double *v, double *F, double *J, double *H, void *stream);
+int smf_hip_eval_matrix_pow(const double *alpha, int64_t n, const double *A, double *F, double *J, double *H,
+                            void *stream);
+int gmf_hip_eval_pow(const double *alpha, int64_t n, const double *A, double *F, double *J, double *H, void *stream);
+/* ... in the mixed form (FP32 outputs) */
+int smf_hip_eval_pow_mixed(const double *alpha, int64_t n, const double *v, float *F, float *J, float *H,
+                           void *stream);
+int smf_hip_eval_matrix_pow_mixed(const double *alpha, int64_t n, const double *A, float *F, float *J, float *H,
+                                  void *stream);
+int gmf_hip_eval_pow_mixed(const double *alpha, int64_t n, const double *A, float *F, float *J, float *H,
+                           void *stream);
+
 /* library version ("0.1.3"-compatible API level plus this build); replaces smf_version (c/include/matfun/matfun.h:53) */
 const char *smf_hip_version(void);
 

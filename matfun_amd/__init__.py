@@ -9,6 +9,10 @@ come back as NumPy arrays).  The mirror of the reference's Python API matfun.smf
 (python/src/matfun/_capi.py:62-113): same argument meaning, shapes and ValueErrors, with one difference inherited
 from the GPU API (cuda/include/matfun/matfun_cuda.h:1-3): a non-symmetric matrix given to smf yields NaN results for
 that matrix instead of an error.  The work runs on torch's current stream, asynchronously.
+
+Extension: for kind "pow", alpha may be an array of the batch shape (one exponent per matrix, any finite value;
+alpha = 0 gives I, alpha < 0 powers of the inverse, a non-finite exponent NaN results for its matrix), which runs the
+*_pow entry points of the C ABI.
 """
 import ctypes
 import math
@@ -39,6 +43,10 @@ def library():
         for fn in (lib.smf_hip_eval, lib.smf_hip_eval_matrix, lib.gmf_hip_eval, lib.smf_hip_eval_mixed,
                    lib.smf_hip_eval_matrix_mixed, lib.gmf_hip_eval_mixed):
             fn.argtypes, fn.restype = args, ctypes.c_int
+        pargs = [ctypes.c_void_p, ctypes.c_int64] + [ctypes.c_void_p] * 5
+        for fn in (lib.smf_hip_eval_pow, lib.smf_hip_eval_matrix_pow, lib.gmf_hip_eval_pow,
+                   lib.smf_hip_eval_pow_mixed, lib.smf_hip_eval_matrix_pow_mixed, lib.gmf_hip_eval_pow_mixed):
+            fn.argtypes, fn.restype = pargs, ctypes.c_int
         lib.smf_hip_version.restype = ctypes.c_char_p
         lib.smf_hip_target.restype = ctypes.c_char_p
         _lib = lib
@@ -56,27 +64,41 @@ def _stream_handle(torch):
     return torch.cuda.current_stream().cuda_stream
 
 
+def _per_matrix(alpha):
+    """True for an exponent array (one per matrix), False for a scalar"""
+    return hasattr(alpha, "shape") and len(alpha.shape) > 0 or isinstance(alpha, (list, tuple))
+
+
 def _prepare(kind, alpha, order):
     if kind not in KINDS:
         raise ValueError(f"unknown function {kind!r}, expected one of {sorted(KINDS)}")
-    if kind == "pow" and not (alpha is not None and alpha > 0):
+    if kind == "pow" and _per_matrix(alpha):
+        pass                                           # (shape checked against the batch in _run)
+    elif kind == "pow" and not (alpha is not None and alpha > 0):
         raise ValueError("pow needs an exponent alpha > 0")
     if order not in (0, 1, 2):
         raise ValueError("order must be 0, 1 or 2")
 
 
-def _run(fn, kind, alpha, A, batch_shape, shapes, order, out_dtype):
+def _run(fn, kind, alpha, A, batch_shape, shapes, order, out_dtype, per_matrix=False):
     import torch
     to_numpy = not isinstance(A, torch.Tensor) or not A.is_cuda
     X = torch.as_tensor(A, dtype=torch.float64)
     if not X.is_cuda:
         X = X.to("cuda")
     X = X.contiguous()
+    n = math.prod(batch_shape)
+    if per_matrix:
+        al = torch.as_tensor(alpha, dtype=torch.float64).to(X.device).contiguous()
+        if tuple(al.shape) != tuple(batch_shape):
+            raise ValueError(f"alpha has shape {tuple(al.shape)}, expected the batch shape {tuple(batch_shape)}")
     out = [torch.empty(batch_shape + s, dtype=out_dtype, device=X.device) for s in shapes[:order + 1]]
     ptrs = [o.data_ptr() for o in out] + [None] * (2 - order)
-    n = math.prod(batch_shape)
     with torch.cuda.device(X.device):
-        r = fn(KINDS[kind], float(alpha or 0.0), n, X.data_ptr() if n else None, *ptrs, _stream_handle(torch))
+        if per_matrix:
+            r = fn(al.data_ptr() if n else None, n, X.data_ptr() if n else None, *ptrs, _stream_handle(torch))
+        else:
+            r = fn(KINDS[kind], float(alpha or 0.0), n, X.data_ptr() if n else None, *ptrs, _stream_handle(torch))
     if r != 0:
         raise ValueError(f"{fn.__name__} returned {r}")
     if to_numpy:
@@ -100,12 +122,15 @@ def smf(kind, A, alpha=None, order=0, precision="fp64"):
     mixed, dt = _precision(precision)
     lib = library()
     shape = tuple(A.shape)
+    pa = kind == "pow" and _per_matrix(alpha)
     if shape[-2:] == (3, 3):
-        return _run(lib.smf_hip_eval_matrix_mixed if mixed else lib.smf_hip_eval_matrix, kind, alpha, A, shape[:-2],
-                    ((3, 3), (3, 3, 3, 3), (3, 3, 3, 3, 3, 3)), order, dt)
+        fn = ((lib.smf_hip_eval_matrix_pow_mixed if mixed else lib.smf_hip_eval_matrix_pow) if pa else
+              (lib.smf_hip_eval_matrix_mixed if mixed else lib.smf_hip_eval_matrix))
+        return _run(fn, kind, alpha, A, shape[:-2], ((3, 3), (3, 3, 3, 3), (3, 3, 3, 3, 3, 3)), order, dt, pa)
     if shape[-1:] == (6,):
-        return _run(lib.smf_hip_eval_mixed if mixed else lib.smf_hip_eval, kind, alpha, A, shape[:-1],
-                    ((6,), (6, 6), (6, 6, 6)), order, dt)
+        fn = ((lib.smf_hip_eval_pow_mixed if mixed else lib.smf_hip_eval_pow) if pa else
+              (lib.smf_hip_eval_mixed if mixed else lib.smf_hip_eval))
+        return _run(fn, kind, alpha, A, shape[:-1], ((6,), (6, 6), (6, 6, 6)), order, dt, pa)
     raise ValueError(f"expected matrices (..., 3, 3) or Voigt vectors (..., 6), got shape {shape}")
 
 
@@ -119,5 +144,7 @@ def gmf(kind, A, alpha=None, order=0, precision="fp64"):
     shape = tuple(A.shape)
     if shape[-2:] != (3, 3):
         raise ValueError(f"expected matrices (..., 3, 3), got shape {shape}")
-    return _run(lib.gmf_hip_eval_mixed if mixed else lib.gmf_hip_eval, kind, alpha, A, shape[:-2],
-                ((3, 3), (3, 3, 3, 3), (3, 3, 3, 3, 3, 3)), order, dt)
+    pa = kind == "pow" and _per_matrix(alpha)
+    fn = ((lib.gmf_hip_eval_pow_mixed if mixed else lib.gmf_hip_eval_pow) if pa else
+          (lib.gmf_hip_eval_mixed if mixed else lib.gmf_hip_eval))
+    return _run(fn, kind, alpha, A, shape[:-2], ((3, 3), (3, 3, 3, 3), (3, 3, 3, 3, 3, 3)), order, dt, pa)

@@ -18,6 +18,12 @@ inline void gmf_coefs(int kind, double al, GmfCoef& c) {
   for (int j = 1; j < GMF_KMAX; j++)
     c.b[j] = kind == EXP ? c.b[j - 1] / j : (kind == LOG ? ((j & 1) ? 1.0 : -1.0) / j : c.b[j - 1] * (al - j + 1) / j);
 }
+// powers with one exponent per matrix: the table holds 1 / j, from which each lane forms its own
+// b_j = b_(j-1) (al - j + 1) / j
+inline void gmf_recips(GmfCoef& c) {
+  c.b[0] = 1.0;
+  for (int j = 1; j < GMF_KMAX; j++) c.b[j] = 1.0 / j;
+}
 
 // the list of divided differences f[x1^a x2^b x3^c], a >= b >= c, ordered by a, then b, then c (19 entries for
 // a <= 3; the first 3 / 9 / 19 serve orders 0 / 1 / 2).  Each entry adds one node to its parent: the first node
@@ -153,17 +159,22 @@ MF_DEV cpx node_value(double al, double sig, double m, cpx x, cpx mu, int k) {
 }
 
 // number of group-series terms (gmf_kernel.hpp:73-97)
-MF_DEV int series_terms(const double* __restrict__ b, int n, double r, double q) {
+// PA: b holds 1 / j and the coefficients binom(al, j) are formed on the way (gmf_recips)
+template <bool PA = false>
+MF_DEV int series_terms(const double* __restrict__ b, double al, int n, double r, double q) {
   int M = 1;
   if (r > 0) {
-    double best = 0.0, c = 1.0, rk = 1.0;
+    double best = 0.0, c = 1.0, rk = 1.0, bn = 1.0;
+    if (PA)
+      for (int j = 1; j <= n; j++) bn = bn * ((al - (j - 1)) * b[j]);
     for (int k = 0; n + k < GMF_KMAX; k++) {
-      const double t = fabs(b[n + k]) * c * rk;
+      const double t = fabs(PA ? bn : b[n + k]) * c * rk;
       best = t > best ? t : best;
       M = k;
       if ((k > 2 && t <= 1e-17 * best) || !(rk > 0)) break;
       c = c * (k + n + 1) / (k + 1);
       rk *= r;
+      if (PA) bn = bn * ((al - (n + k)) * b[n + k + 1 < GMF_KMAX ? n + k + 1 : GMF_KMAX - 1]);
     }
   }
   if (q > 0) {
@@ -186,7 +197,8 @@ struct GmfOut {
   double sc1, sc2;    // factors of J and H (NaN for a matrix outside the domain)
 };
 
-template <int KIND, int ORDER>
+// PA (powers with a per-matrix exponent al): bt holds 1 / j instead of the coefficients b_j (gmf_recips)
+template <int KIND, int ORDER, bool PA = false>
 MF_DEV void gmf_lane(const double* a, double al, const double* __restrict__ bt, GmfOut& o) {
   constexpr int ND = ORDER == 0 ? 3 : (ORDER == 1 ? 9 : 19);
   const double RHO = 0.7, R_EXP = 4.0, RHO_POLE = 0.6;
@@ -418,7 +430,7 @@ MF_DEV void gmf_lane(const double* a, double al, const double* __restrict__ bt, 
 #pragma unroll
   for (int c = 0; c < ND; c++) D[c] = mk(0.0, 0.0);
   if (hasg) {
-    const int K = series_terms(bt, 3 * (ORDER + 1) - 1, rr, rq);
+    const int K = series_terms<PA>(bt, al, 3 * (ORDER + 1) - 1, rr, rq);
     const double b0 = KIND == LOG ? node_value<LOG>(al, sig, m, mk(cxg, 0.0), mk(cmu, 0.0), 0).re : 1.0;
     cpx E[ND], acc[ND];
 #pragma unroll
@@ -431,7 +443,8 @@ MF_DEV void gmf_lane(const double* a, double al, const double* __restrict__ bt, 
     double bnext = 0.0;                         // (b_j loaded one iteration ahead of its use)
     for (int j = 0; j < K; j++) {
       const double bj = j == 0 ? b0 : bnext;
-      bnext = bt[j + 1 < GMF_KMAX ? j + 1 : GMF_KMAX - 1];
+      bnext = PA ? bj * ((al - j) * bt[j + 1 < GMF_KMAX ? j + 1 : GMF_KMAX - 1])
+                 : bt[j + 1 < GMF_KMAX ? j + 1 : GMF_KMAX - 1];
       const double g1 = (bj - q1) * iap, g2 = (g1 - q2) * iap, g3 = (g2 - q3) * iap;
       q1 = g1; q2 = g2; q3 = g3;
       // E in place: children after their parents in the list, so a descending pass sees the parents' values at

@@ -5,7 +5,8 @@
 //            lane) into an LDS tile in the API layout, which the wave then writes as one contiguous block (2 x 5.8 KB
 //            of consecutive output per pass; whole cache lines per store instruction)
 // OT = double: the FP64 form; OT = float: the mixed form (FP64 per-lane math, FP32 staged tensors, J / H assembly
-// and outputs).
+// and outputs).  PA: powers with one exponent per matrix (alv[i], read by its lane; bt then holds 1 / j; a matrix
+// whose exponent is not finite gets NaN results).
 #include "gmf_device.hpp"
 #include "smf_launch.hpp"
 
@@ -25,9 +26,10 @@ struct GmfLds {
   static constexpr int MAIN = ORDER == 0 ? 0 : 64 * MS;
 };
 
-template <int KIND, int ORDER, class OT>
+template <int KIND, int ORDER, class OT, bool PA>
 __global__ void __launch_bounds__(GWAVE) gmf_kernel(int64_t n, const double* __restrict__ A, OT* __restrict__ F,
-                                                    OT* __restrict__ J, OT* __restrict__ H, double al, GmfCoef bt) {
+                                                    OT* __restrict__ J, OT* __restrict__ H, double al,
+                                                    const double* __restrict__ alv, GmfCoef bt) {
   using L = GmfLds<ORDER>;
   constexpr int TOT = L::MAIN + L::XS;
   constexpr int BYTES = (TOT * (int)sizeof(OT) > GWAVE * 9 * 8) ? TOT * (int)sizeof(OT) : GWAVE * 9 * 8;
@@ -46,9 +48,23 @@ __global__ void __launch_bounds__(GWAVE) gmf_kernel(int64_t n, const double* __r
   for (int k = 0; k < 9; k++) a[k] = lane < nm ? xin[9 * lane + k] : (k % 4 == 0 ? 1.0 : 0.0);
   __syncthreads();
 
+  bool alok = true;                                  // (PA: a non-finite exponent computes with 1 and stores NaN)
+  if (PA) {
+    al = lane < nm ? alv[base + lane] : 1.0;
+    alok = fabs(al) <= 1.7976931348623157e308;
+    al = alok ? al : 1.0;
+  }
+
   // ---- phase 1
   GmfOut o;
-  gmf_lane<KIND, ORDER>(a, al, bt.b, o);
+  gmf_lane<KIND, ORDER, PA>(a, al, bt.b, o);
+  if (PA) {                                          // (J and H carry the factors sc1, sc2)
+    const double nanf = alok ? 1.0 : dnan();
+#pragma unroll
+    for (int k = 0; k < 9; k++) o.F[k] *= nanf;
+    o.sc1 *= nanf;
+    o.sc2 *= nanf;
+  }
   if (ORDER >= 1) {
     OT* my = mainr + lane * L::MS;
 #pragma unroll
@@ -120,12 +136,15 @@ __global__ void __launch_bounds__(GWAVE) gmf_kernel(int64_t n, const double* __r
   }
 }
 
-template <int KIND, class OT>
+template <int KIND, class OT, bool PA>
 static void glaunch_kind(hipStream_t st, unsigned blocks, int64_t n, const double* A, OT* F, OT* J, OT* H, double al,
-                         const GmfCoef& bt) {
-  if (H) hipLaunchKernelGGL((gmf_kernel<KIND, 2, OT>), dim3(blocks), dim3(GWAVE), 0, st, n, A, F, J, H, al, bt);
-  else if (J) hipLaunchKernelGGL((gmf_kernel<KIND, 1, OT>), dim3(blocks), dim3(GWAVE), 0, st, n, A, F, J, H, al, bt);
-  else hipLaunchKernelGGL((gmf_kernel<KIND, 0, OT>), dim3(blocks), dim3(GWAVE), 0, st, n, A, F, J, H, al, bt);
+                         const double* alv, const GmfCoef& bt) {
+  if (H)
+    hipLaunchKernelGGL((gmf_kernel<KIND, 2, OT, PA>), dim3(blocks), dim3(GWAVE), 0, st, n, A, F, J, H, al, alv, bt);
+  else if (J)
+    hipLaunchKernelGGL((gmf_kernel<KIND, 1, OT, PA>), dim3(blocks), dim3(GWAVE), 0, st, n, A, F, J, H, al, alv, bt);
+  else
+    hipLaunchKernelGGL((gmf_kernel<KIND, 0, OT, PA>), dim3(blocks), dim3(GWAVE), 0, st, n, A, F, J, H, al, alv, bt);
 }
 
 template <class OT>
@@ -133,10 +152,17 @@ static void glaunch_any(hipStream_t st, int kind, int64_t n, const double* A, OT
                         const GmfCoef& bt) {
   const unsigned blocks = (unsigned)((n + GWAVE - 1) / GWAVE);
   switch (kind) {
-    case EXP: glaunch_kind<EXP, OT>(st, blocks, n, A, F, J, H, al, bt); break;
-    case LOG: glaunch_kind<LOG, OT>(st, blocks, n, A, F, J, H, al, bt); break;
-    default: glaunch_kind<POW, OT>(st, blocks, n, A, F, J, H, al, bt); break;
+    case EXP: glaunch_kind<EXP, OT, false>(st, blocks, n, A, F, J, H, al, nullptr, bt); break;
+    case LOG: glaunch_kind<LOG, OT, false>(st, blocks, n, A, F, J, H, al, nullptr, bt); break;
+    default: glaunch_kind<POW, OT, false>(st, blocks, n, A, F, J, H, al, nullptr, bt); break;
   }
+}
+
+template <class OT>
+static void glaunch_pow(hipStream_t st, int64_t n, const double* alv, const double* A, OT* F, OT* J, OT* H,
+                        const GmfCoef& recips) {
+  const unsigned blocks = (unsigned)((n + GWAVE - 1) / GWAVE);
+  glaunch_kind<POW, OT, true>(st, blocks, n, A, F, J, H, 1.0, alv, recips);
 }
 
 void gmf_launch(hipStream_t st, int kind, int64_t n, const double* A, double* F, double* J, double* H, double al,
@@ -147,6 +173,16 @@ void gmf_launch(hipStream_t st, int kind, int64_t n, const double* A, double* F,
 void gmf_launch_mixed(hipStream_t st, int kind, int64_t n, const double* A, float* F, float* J, float* H, double al,
                       const GmfCoef& bt) {
   glaunch_any<float>(st, kind, n, A, F, J, H, al, bt);
+}
+
+void gmf_launch_pow(hipStream_t st, int64_t n, const double* alv, const double* A, double* F, double* J, double* H,
+                    const GmfCoef& recips) {
+  glaunch_pow<double>(st, n, alv, A, F, J, H, recips);
+}
+
+void gmf_launch_pow_mixed(hipStream_t st, int64_t n, const double* alv, const double* A, float* F, float* J, float* H,
+                          const GmfCoef& recips) {
+  glaunch_pow<float>(st, n, alv, A, F, J, H, recips);
 }
 
 }  // namespace mf

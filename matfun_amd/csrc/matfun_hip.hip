@@ -45,6 +45,22 @@ void glaunch_chunk(hipStream_t st, int kind, int64_t n, const double* A, float* 
   mf::gmf_launch_mixed(st, kind, n, A, F, J, H, al, bt);
 }
 
+void pow_chunk(hipStream_t st, bool mat, int64_t n, const double* alv, const double* x, double* F, double* J,
+               double* H) {
+  mf::smf_launch_pow(st, mat, n, alv, x, F, J, H);
+}
+void pow_chunk(hipStream_t st, bool mat, int64_t n, const double* alv, const double* x, float* F, float* J, float* H) {
+  mf::smf_launch_pow_mixed(st, mat, n, alv, x, F, J, H);
+}
+void gpow_chunk(hipStream_t st, int64_t n, const double* alv, const double* A, double* F, double* J, double* H,
+                const mf::GmfCoef& r) {
+  mf::gmf_launch_pow(st, n, alv, A, F, J, H, r);
+}
+void gpow_chunk(hipStream_t st, int64_t n, const double* alv, const double* A, float* F, float* J, float* H,
+                const mf::GmfCoef& r) {
+  mf::gmf_launch_pow_mixed(st, n, alv, A, F, J, H, r);
+}
+
 // symmetric matrices (x: Voigt vectors or full matrices); splits batches beyond one grid into several launches
 template <class OT>
 int smf_call(int kind, double alpha, int64_t n, const double* x, bool mat, OT* F, OT* J, OT* H, void* stream) {
@@ -83,6 +99,38 @@ int gmf_call(int kind, double alpha, int64_t n, const double* A, OT* F, OT* J, O
   return 0;
 }
 
+// powers with one exponent per matrix (device array alpha of n doubles, any finite value)
+template <class OT>
+int smf_pow_call(const double* alpha, int64_t n, const double* x, bool mat, OT* F, OT* J, OT* H, void* stream) {
+  if (n < 0 || (H && !J) || (n > 0 && !alpha)) return -1;
+  if (n == 0) return 0;
+  if (n > SMF_MAX_N) return -2;
+  const int nin = mat ? 9 : 6, nf = mat ? 9 : 6;
+  for (int64_t lo = 0; lo < n; lo += mf::MAX_PER_LAUNCH) {
+    const int64_t m = n - lo < mf::MAX_PER_LAUNCH ? n - lo : mf::MAX_PER_LAUNCH;
+    pow_chunk((hipStream_t)stream, mat, m, alpha + lo, x + nin * lo, F + nf * lo, J ? J + nf * nf * lo : nullptr,
+              H ? H + nf * nf * nf * lo : nullptr);
+    if (launched()) return -2;
+  }
+  return 0;
+}
+
+template <class OT>
+int gmf_pow_call(const double* alpha, int64_t n, const double* A, OT* F, OT* J, OT* H, void* stream) {
+  if (n < 0 || (H && !J) || (n > 0 && !alpha)) return -1;
+  if (n == 0) return 0;
+  if (n > GMF_MAX_N) return -2;
+  mf::GmfCoef recips;
+  mf::gmf_recips(recips);
+  for (int64_t lo = 0; lo < n; lo += mf::MAX_PER_LAUNCH) {
+    const int64_t m = n - lo < mf::MAX_PER_LAUNCH ? n - lo : mf::MAX_PER_LAUNCH;
+    gpow_chunk((hipStream_t)stream, m, alpha + lo, A + 9 * lo, F + 9 * lo, J ? J + 81 * lo : nullptr,
+               H ? H + 729 * lo : nullptr, recips);
+    if (launched()) return -2;
+  }
+  return 0;
+}
+
 }  // namespace
 
 extern "C" int smf_hip_eval(int kind, double alpha, int64_t n, const double* v, double* F, double* J, double* H,
@@ -113,6 +161,36 @@ extern "C" int smf_hip_eval_matrix_mixed(int kind, double alpha, int64_t n, cons
 extern "C" int gmf_hip_eval_mixed(int kind, double alpha, int64_t n, const double* A, float* F, float* J, float* H,
                                   void* stream) {
   return gmf_call(kind, alpha, n, A, F, J, H, stream);
+}
+
+extern "C" int smf_hip_eval_pow(const double* alpha, int64_t n, const double* v, double* F, double* J, double* H,
+                                void* stream) {
+  return smf_pow_call(alpha, n, v, false, F, J, H, stream);
+}
+
+extern "C" int smf_hip_eval_matrix_pow(const double* alpha, int64_t n, const double* A, double* F, double* J,
+                                       double* H, void* stream) {
+  return smf_pow_call(alpha, n, A, true, F, J, H, stream);
+}
+
+extern "C" int gmf_hip_eval_pow(const double* alpha, int64_t n, const double* A, double* F, double* J, double* H,
+                                void* stream) {
+  return gmf_pow_call(alpha, n, A, F, J, H, stream);
+}
+
+extern "C" int smf_hip_eval_pow_mixed(const double* alpha, int64_t n, const double* v, float* F, float* J, float* H,
+                                      void* stream) {
+  return smf_pow_call(alpha, n, v, false, F, J, H, stream);
+}
+
+extern "C" int smf_hip_eval_matrix_pow_mixed(const double* alpha, int64_t n, const double* A, float* F, float* J,
+                                             float* H, void* stream) {
+  return smf_pow_call(alpha, n, A, true, F, J, H, stream);
+}
+
+extern "C" int gmf_hip_eval_pow_mixed(const double* alpha, int64_t n, const double* A, float* F, float* J, float* H,
+                                      void* stream) {
+  return gmf_pow_call(alpha, n, A, F, J, H, stream);
 }
 
 extern "C" const char* smf_hip_version(void) { return MATFUN_HIP_VERSION; }

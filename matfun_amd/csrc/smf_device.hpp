@@ -74,8 +74,10 @@ MF_DEV double dfun(double x, double fx, double al) {
 }
 
 // f[x, y, z] (smf_kernel.hpp:148-192): Taylor series around the middle point inside a cluster, else the difference
-// formula over the most distant pair
-template <int KIND>
+// formula over the most distant pair.  PA (powers with a per-matrix exponent): the series coefficients
+// binom(al, k + 2) are formed in the loop from binom(al, 2) = al (al - 1) / 2 by c_k = c_(k-1) (al - k - 1) / (k + 2),
+// the recurrence of smf_coefs, and c is not read
+template <int KIND, bool PA = false>
 MF_DEV double dd2(double li, double lj, double lk, double fi, double fj, double fk, double Dij, double Dik, double Djk,
                   double al, const double* __restrict__ c) {
   constexpr bool powlike = KIND == POW || KIND == SQRT;
@@ -112,12 +114,13 @@ MF_DEV double dd2(double li, double lj, double lk, double fi, double fj, double 
     scl = (KIND == LOG ? 1.0 : (cluster ? fm : 1.0)) / (ms * ms);
   }
   if (cluster) {                       // h_k(u, w) = u h_(k-1) + w^k
-    double h = 1.0, wk = 1.0, s = c[0];
+    double h = 1.0, wk = 1.0, ck = PA ? al * (al - 1.0) * 0.5 : c[0], s = ck;
 #pragma unroll
     for (int k = 1; k < SMF_NTERMS; k++) {
       wk = wk * w;
       h = u * h + wk;
-      s = s + c[k] * h;
+      ck = PA ? ck * ((al - (k + 1.0)) * (1.0 / (k + 2))) : c[k];
+      s = s + ck * h;
     }
     out = scl * s;
   }
@@ -267,6 +270,7 @@ struct SmfLane {
   }
 
   // ---- the 10 second divided differences (smf_kernel.hpp:330-338)
+  template <bool PA = false>
   MF_DEV void second_dd(double al, const double* __restrict__ c2) {
 #pragma unroll
     for (int i = 0; i < 3; i++)
@@ -274,7 +278,8 @@ struct SmfLane {
       for (int j = i; j < 3; j++)
 #pragma unroll
         for (int k = j; k < 3; k++)
-          D2[d2idx(i, j, k)] = dd2<KIND>(lam[i], lam[j], lam[k], fl[i], fl[j], fl[k], D[i][j], D[i][k], D[j][k], al, c2);
+          D2[d2idx(i, j, k)] =
+              dd2<KIND, PA>(lam[i], lam[j], lam[k], fl[i], fl[j], fl[k], D[i][j], D[i][k], D[j][k], al, c2);
   }
 };
 

@@ -7,7 +7,8 @@
 // Every store instruction of the wave writes 64 consecutive elements: the output (96 B .. 5.8 KB per matrix) bounds
 // the kernel by HBM write bandwidth, which needs whole cache lines per instruction.
 // OT = double: the FP64 form; OT = float: the mixed form (FP64 eigen-decomposition and divided differences, FP32
-// derivative assembly and FP32 outputs).
+// derivative assembly and FP32 outputs).  PA: powers with one exponent per matrix (alv[i], read by its lane; a
+// matrix whose exponent is not finite gets NaN results).
 #include "smf_device.hpp"
 #include "smf_launch.hpp"
 
@@ -44,9 +45,10 @@ MF_DEV void copy_out(OT* __restrict__ out, int count, int lane, Get get) {
   }
 }
 
-template <int KIND, bool MAT, int ORDER, class OT, bool PAIRS>
+template <int KIND, bool MAT, int ORDER, class OT, bool PAIRS, bool PA>
 __global__ void __launch_bounds__(WAVE) smf_kernel(int64_t n, const double* __restrict__ x, OT* __restrict__ F,
-                                                   OT* __restrict__ J, OT* __restrict__ H, double al, SmfCoef c2) {
+                                                   OT* __restrict__ J, OT* __restrict__ H, double al,
+                                                   const double* __restrict__ alv, SmfCoef c2) {
   constexpr int NIN = MAT ? 9 : 6, NF = MAT ? 9 : 6, NJ = NF * NF, NH = NJ * NF;
   constexpr int R1 = (WAVE * SA > HPP * SH ? WAVE * SA : HPP * SH);             // region 1 (OT): phase-A / H tiles
   constexpr int R1B = (R1 * (int)sizeof(OT) > WAVE * NIN * 8 ? R1 * (int)sizeof(OT) : WAVE * NIN * 8);
@@ -87,6 +89,13 @@ __global__ void __launch_bounds__(WAVE) smf_kernel(int64_t n, const double* __re
     for (int k = 0; k < 6; k++) v[k] = lane < nm ? xin[6 * lane + k] : (k < 3 ? 1.0 : 0.0);
   }
   __syncthreads();
+  bool alok = true;                            // (PA: a non-finite exponent computes with 1 and stores NaN)
+  if (PA) {
+    al = lane < nm ? alv[base + lane] : 1.0;
+    alok = fabs(al) <= 1.7976931348623157e308;
+    al = alok ? al : 1.0;
+  }
+  const double nanf = alok ? 1.0 : dnan();
 
   // ---- phase A
   {
@@ -94,19 +103,19 @@ __global__ void __launch_bounds__(WAVE) smf_kernel(int64_t n, const double* __re
     double Fv[6];
     s.eigen(v, al, Fv);
 #pragma unroll
-    for (int k = 0; k < 6; k++) t1[lane * SA + k] = (OT)Fv[k];
+    for (int k = 0; k < 6; k++) t1[lane * SA + k] = (OT)(Fv[k] * nanf);
     if (ORDER >= 1) {
       OT Js[21];
       s.jacobian(al, Js);
 #pragma unroll
-      for (int k = 0; k < 21; k++) t1[lane * SA + 6 + k] = Js[k];
+      for (int k = 0; k < 21; k++) t1[lane * SA + 6 + k] = Js[k] * (OT)nanf;
     }
     if (ORDER >= 2) {
-      s.second_dd(al, c2.c);
+      s.template second_dd<PA>(al, c2.c);
 #pragma unroll
       for (int k = 0; k < 9; k++) tb[lane * SB + k] = (OT)s.V[k / 3][k % 3];
 #pragma unroll
-      for (int k = 0; k < 10; k++) tb[lane * SB + 9 + k] = (OT)s.D2[k];
+      for (int k = 0; k < 10; k++) tb[lane * SB + 9 + k] = (OT)(s.D2[k] * nanf);
     }
   }
   __syncthreads();
@@ -144,33 +153,39 @@ __global__ void __launch_bounds__(WAVE) smf_kernel(int64_t n, const double* __re
   }
 }
 
-template <int KIND, bool MAT, class OT, bool PAIRS>
+template <int KIND, bool MAT, class OT, bool PAIRS, bool PA>
 static void launch_order(hipStream_t st, unsigned blocks, int64_t n, const double* x, OT* F, OT* J, OT* H, double al,
-                         const SmfCoef& c2) {
-  if (H) hipLaunchKernelGGL((smf_kernel<KIND, MAT, 2, OT, PAIRS>), dim3(blocks), dim3(WAVE), 0, st, n, x, F, J, H, al, c2);
-  else if (J) hipLaunchKernelGGL((smf_kernel<KIND, MAT, 1, OT, PAIRS>), dim3(blocks), dim3(WAVE), 0, st, n, x, F, J, H, al, c2);
-  else hipLaunchKernelGGL((smf_kernel<KIND, MAT, 0, OT, false>), dim3(blocks), dim3(WAVE), 0, st, n, x, F, J, H, al, c2);
+                         const double* alv, const SmfCoef& c2) {
+  if (H)
+    hipLaunchKernelGGL((smf_kernel<KIND, MAT, 2, OT, PAIRS, PA>), dim3(blocks), dim3(WAVE), 0, st, n, x, F, J, H, al,
+                       alv, c2);
+  else if (J)
+    hipLaunchKernelGGL((smf_kernel<KIND, MAT, 1, OT, PAIRS, PA>), dim3(blocks), dim3(WAVE), 0, st, n, x, F, J, H, al,
+                       alv, c2);
+  else
+    hipLaunchKernelGGL((smf_kernel<KIND, MAT, 0, OT, false, PA>), dim3(blocks), dim3(WAVE), 0, st, n, x, F, J, H, al,
+                       alv, c2);
 }
 
 // paired stores need J and H aligned to two elements (the per-wave regions then stay aligned: 64 and 10 matrices of
 // 81 / 729 (36 / 216) elements are even counts)
-template <int KIND, bool MAT, class OT>
+template <int KIND, bool MAT, class OT, bool PA>
 static void launch_kind(hipStream_t st, unsigned blocks, int64_t n, const double* x, OT* F, OT* J, OT* H, double al,
-                        const SmfCoef& c2) {
+                        const double* alv, const SmfCoef& c2) {
   const uintptr_t al2 = 2 * sizeof(OT) - 1;
   if ((!J || ((uintptr_t)J & al2) == 0) && (!H || ((uintptr_t)H & al2) == 0))
-    launch_order<KIND, MAT, OT, true>(st, blocks, n, x, F, J, H, al, c2);
+    launch_order<KIND, MAT, OT, true, PA>(st, blocks, n, x, F, J, H, al, alv, c2);
   else
-    launch_order<KIND, MAT, OT, false>(st, blocks, n, x, F, J, H, al, c2);
+    launch_order<KIND, MAT, OT, false, PA>(st, blocks, n, x, F, J, H, al, alv, c2);
 }
 
 template <class OT>
 static void launch_any(hipStream_t st, int kind, bool mat, int64_t n, const double* x, OT* F, OT* J, OT* H, double al,
                        const SmfCoef& c2) {
   const unsigned blocks = (unsigned)((n + WAVE - 1) / WAVE);
-#define MF_CASE(K)                                                             \
-  if (mat) launch_kind<K, true, OT>(st, blocks, n, x, F, J, H, al, c2);        \
-  else launch_kind<K, false, OT>(st, blocks, n, x, F, J, H, al, c2);
+#define MF_CASE(K)                                                                        \
+  if (mat) launch_kind<K, true, OT, false>(st, blocks, n, x, F, J, H, al, nullptr, c2);   \
+  else launch_kind<K, false, OT, false>(st, blocks, n, x, F, J, H, al, nullptr, c2);
   switch (kind) {
     case EXP: MF_CASE(EXP) break;
     case LOG: MF_CASE(LOG) break;
@@ -178,6 +193,14 @@ static void launch_any(hipStream_t st, int kind, bool mat, int64_t n, const doub
     default: MF_CASE(POW) break;
   }
 #undef MF_CASE
+}
+
+template <class OT>
+static void launch_pow(hipStream_t st, bool mat, int64_t n, const double* alv, const double* x, OT* F, OT* J, OT* H) {
+  const unsigned blocks = (unsigned)((n + WAVE - 1) / WAVE);
+  const SmfCoef unused{};
+  if (mat) launch_kind<POW, true, OT, true>(st, blocks, n, x, F, J, H, 1.0, alv, unused);
+  else launch_kind<POW, false, OT, true>(st, blocks, n, x, F, J, H, 1.0, alv, unused);
 }
 
 void smf_launch(hipStream_t st, int kind, bool mat, int64_t n, const double* x, double* F, double* J, double* H,
@@ -188,6 +211,16 @@ void smf_launch(hipStream_t st, int kind, bool mat, int64_t n, const double* x, 
 void smf_launch_mixed(hipStream_t st, int kind, bool mat, int64_t n, const double* x, float* F, float* J, float* H,
                       double al, const SmfCoef& c2) {
   launch_any<float>(st, kind, mat, n, x, F, J, H, al, c2);
+}
+
+void smf_launch_pow(hipStream_t st, bool mat, int64_t n, const double* alv, const double* x, double* F, double* J,
+                    double* H) {
+  launch_pow<double>(st, mat, n, alv, x, F, J, H);
+}
+
+void smf_launch_pow_mixed(hipStream_t st, bool mat, int64_t n, const double* alv, const double* x, float* F, float* J,
+                          float* H) {
+  launch_pow<float>(st, mat, n, alv, x, F, J, H);
 }
 
 }  // namespace mf

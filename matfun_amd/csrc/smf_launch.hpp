@@ -19,6 +19,16 @@ void smf_launch_mixed(hipStream_t st, int kind, bool mat, int64_t n, const doubl
 void gmf_launch_mixed(hipStream_t st, int kind, int64_t n, const double* A, float* F, float* J, float* H, double al,
                       const GmfCoef& bt);
 
+// powers with one exponent per matrix (alv: n device doubles); gmf: recips from gmf_recips
+void smf_launch_pow(hipStream_t st, bool mat, int64_t n, const double* alv, const double* x, double* F, double* J,
+                    double* H);
+void smf_launch_pow_mixed(hipStream_t st, bool mat, int64_t n, const double* alv, const double* x, float* F, float* J,
+                          float* H);
+void gmf_launch_pow(hipStream_t st, int64_t n, const double* alv, const double* A, double* F, double* J, double* H,
+                    const GmfCoef& recips);
+void gmf_launch_pow_mixed(hipStream_t st, int64_t n, const double* alv, const double* A, float* F, float* J, float* H,
+                          const GmfCoef& recips);
+
 constexpr int64_t MAX_PER_LAUNCH = (int64_t)1 << 30;   // matrices per grid (work-items stay below 2^32)
 
 }  // namespace mf

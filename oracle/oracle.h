@@ -12,7 +12,8 @@
  *   oracle_gmf_eval        general real matrices, matrix layout (c/include/matfun/gmf_kernel.hpp)
  * Returns 0, -1 (invalid arguments, c/src/matfun.cc:23-27) or -3.
  *
- * Extension (no counterpart in the reference API): oracle_smf_eval_matrix_alpha / oracle_gmf_eval_alpha take one
+ * Extension (no counterpart in the reference API): oracle_smf_eval_alpha / oracle_smf_eval_matrix_alpha /
+ * oracle_gmf_eval_alpha take one
  * exponent per matrix (pow only; any finite value -- the reference requires alpha > 0; for alpha <= 0 the series
  * radii 3/alpha become 3/|alpha|, identical for alpha > 0) and never return -3 (NaN results instead, as the GPU
  * API does).
@@ -30,6 +31,7 @@ int oracle_smf_eval_matrix(int kind, double alpha, int64_t n, const double *A, d
 int oracle_gmf_eval(int kind, double alpha, int64_t n, const double *A, double *F, double *J, double *H);
 
 /* per-matrix exponents (kind must be 2 = pow; every alpha[i] finite) */
+int oracle_smf_eval_alpha(int kind, const double *alpha, int64_t n, const double *v, double *F, double *J, double *H);
 int oracle_smf_eval_matrix_alpha(int kind, const double *alpha, int64_t n, const double *A, double *F, double *J,
                                  double *H);
 int oracle_gmf_eval_alpha(int kind, const double *alpha, int64_t n, const double *A, double *F, double *J, double *H);

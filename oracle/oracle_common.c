@@ -34,6 +34,19 @@ int oracle_smf_eval(int kind, double alpha, int64_t n, const double *v, double *
   return 0;
 }
 
+int oracle_smf_eval_alpha(int kind, const double *alpha, int64_t n, const double *v, double *F, double *J, double *H) {
+  if (kind != K_POW || n < 0 || (H && !J)) return -1;
+  for (int64_t i = 0; i < n; i++)
+    if (!isfinite(alpha[i])) return -1;
+#pragma omp parallel for schedule(static) num_threads(nthreads_for(n))
+  for (int64_t i = 0; i < n; i++) {
+    double c2[SMF_NTERMS];
+    smf_dd2_coefs(kind, alpha[i], c2);
+    smf_eval_one(kind, alpha[i], v + 6 * i, F + 6 * i, J ? J + 36 * i : NULL, H ? H + 216 * i : NULL, c2);
+  }
+  return 0;
+}
+
 static int smf_matrix_batch(int kind, const double *alphas, double alpha, int64_t n, const double *A, double *F,
                             double *J, double *H) {
   int asym = 0;

@@ -39,7 +39,8 @@ class Checker:
         if which == "oracle":
             aargs = [ctypes.c_int, ctypes.c_void_p, ctypes.c_int64] + [ctypes.c_void_p] * 4
             self._smfa, self._gmfa = self.lib.oracle_smf_eval_matrix_alpha, self.lib.oracle_gmf_eval_alpha
-            for f in (self._smfa, self._gmfa):
+            self._smfva = self.lib.oracle_smf_eval_alpha
+            for f in (self._smfa, self._gmfa, self._smfva):
                 f.argtypes, f.restype = aargs, ctypes.c_int
             self.lib.oracle_set_threads.argtypes = [ctypes.c_int]
             self.lib.oracle_gmf_terms.argtypes = [ctypes.c_int, ctypes.c_double, ctypes.c_int, ctypes.c_void_p]
@@ -79,17 +80,20 @@ class Checker:
     def gmf(self, kind, alpha, A, order=2):
         return self._run(self._gmf, kind, alpha, A, 9, 9, order)
 
-    def _run_alpha(self, fn, alphas, X, order):
-        X = np.ascontiguousarray(X, dtype=np.float64).reshape(-1, 9)
+    def _run_alpha(self, fn, alphas, X, order, nx=9):
+        X = np.ascontiguousarray(X, dtype=np.float64).reshape(-1, nx)
         alphas = np.ascontiguousarray(alphas, dtype=np.float64)
         n = X.shape[0]
         assert alphas.shape == (n,)
-        F = np.empty((n, 9))
-        J = np.empty((n, 81)) if order >= 1 else None
-        H = np.empty((n, 729)) if order >= 2 else None
+        F = np.empty((n, nx))
+        J = np.empty((n, nx * nx)) if order >= 1 else None
+        H = np.empty((n, nx ** 3)) if order >= 2 else None
         p = lambda a: None if a is None else a.ctypes.data
         rc = fn(2, alphas.ctypes.data, n, X.ctypes.data, F.ctypes.data, p(J), p(H))
         return rc, F, J, H
+
+    def smf_alpha(self, alphas, v, order=2):
+        return self._run_alpha(self._smfva, alphas, v, order, nx=6)
 
     def smf_matrix_alpha(self, alphas, A, order=2):
         return self._run_alpha(self._smfa, alphas, A, order)

@@ -31,3 +31,29 @@ def call(family, kind, alpha, X, order, stream=None, mixed=False):
     torch.cuda.synchronize()
     g = lambda t: None if t is None else t.cpu().double().numpy()
     return rc, g(dF), g(dJ), g(dH)
+
+
+def call_pow(family, alphas, X, order, mixed=False):
+    """run smf_hip_eval_pow / smf_hip_eval_matrix_pow / gmf_hip_eval_pow (or their _mixed forms) with one exponent
+    per matrix; returns rc, F, J, H (NumPy)"""
+    import torch
+    lib = matfun_amd.library()
+    fn = {("voigt", False): lib.smf_hip_eval_pow, ("matrix", False): lib.smf_hip_eval_matrix_pow,
+          ("general", False): lib.gmf_hip_eval_pow, ("voigt", True): lib.smf_hip_eval_pow_mixed,
+          ("matrix", True): lib.smf_hip_eval_matrix_pow_mixed, ("general", True): lib.gmf_hip_eval_pow_mixed}[
+        (family, mixed)]
+    dt = torch.float32 if mixed else torch.float64
+    nin, nf = SIZES[family]
+    X = np.ascontiguousarray(X, dtype=np.float64).reshape(-1, nin)
+    n = X.shape[0]
+    assert len(alphas) == n
+    dX = torch.from_numpy(X).to("cuda")
+    dA = torch.from_numpy(np.ascontiguousarray(alphas, dtype=np.float64)).to("cuda")
+    dF = torch.full((n, nf), 7.0, dtype=dt, device="cuda")
+    dJ = torch.full((n, nf * nf), 7.0, dtype=dt, device="cuda") if order >= 1 else None
+    dH = torch.full((n, nf ** 3), 7.0, dtype=dt, device="cuda") if order >= 2 else None
+    p = lambda t: None if t is None else t.data_ptr()
+    rc = fn(p(dA), n, p(dX), p(dF), p(dJ), p(dH), torch.cuda.current_stream().cuda_stream)
+    torch.cuda.synchronize()
+    g = lambda t: None if t is None else t.cpu().double().numpy()
+    return rc, g(dF), g(dJ), g(dH)

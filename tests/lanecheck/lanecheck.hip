@@ -8,29 +8,41 @@
 
 namespace {
 
-// one matrix per call of the lane math; the outputs are gathered through the kernels' index maps (f_src, j_src, h_src)
-template <int KIND>
-void smf_any(bool mat, double al, int64_t n, const double* x, double* F, double* J, double* H) {
+// a per-matrix exponent as the kernels take it: a non-finite one computes with 1 and stores NaN
+double exponent_of(const double* alv, int64_t m, double& nanf) {
+  const double a = alv[m];
+  const bool ok = fabs(a) <= 1.7976931348623157e308;
+  nanf = ok ? 1.0 : mf::dnan();
+  return ok ? a : 1.0;
+}
+
+// one matrix per call of the lane math; the outputs are gathered through the kernels' index maps (f_src, j_src, h_src).
+// PA: powers with the per-matrix exponents alv
+template <int KIND, bool PA = false>
+void smf_any(bool mat, double al, const double* alv, int64_t n, const double* x, double* F, double* J, double* H) {
   mf::SmfCoef c2;
   mf::smf_coefs(KIND, al, c2);
   const int nf = mat ? 9 : 6, nj = nf * nf, nh = nj * nf;
   for (int64_t m = 0; m < n; m++) {
+    double nanf = 1.0;
+    if (PA) al = exponent_of(alv, m, nanf);
     double v[6];
     if (mat) mf::matrix_to_voigt(x + 9 * m, v);
     else for (int k = 0; k < 6; k++) v[k] = x[6 * m + k];
     mf::SmfLane<KIND, double> s;
     double Fv[6], Js[21], Hv[126];
     s.eigen(v, al, Fv);
-    for (int e = 0; e < nf; e++) F[nf * m + e] = Fv[mf::f_src(mat, e)];
+    for (int e = 0; e < nf; e++) F[nf * m + e] = Fv[mf::f_src(mat, e)] * nanf;
     if (!J) continue;
     s.jacobian(al, Js);
     for (int e = 0; e < nj; e++) {
       double w;
       const int src = mf::j_src(mat, e, w);
-      J[nj * m + e] = Js[src] * w;
+      J[nj * m + e] = Js[src] * nanf * w;
     }
     if (!H) continue;
-    s.second_dd(al, c2.c);
+    s.template second_dd<PA>(al, c2.c);
+    for (int k = 0; k < 10; k++) s.D2[k] *= nanf;
     double V9[9];
     for (int k = 0; k < 9; k++) V9[k] = s.V[k / 3][k % 3];
     const mf::SmfHess<double> hs(V9, s.D2);
@@ -43,13 +55,19 @@ void smf_any(bool mat, double al, int64_t n, const double* x, double* F, double*
   }
 }
 
-template <int KIND, int ORDER>
-void gmf_all(double al, int64_t n, const double* A, double* F, double* J, double* H) {
+template <int KIND, int ORDER, bool PA = false>
+void gmf_all(double al, const double* alv, int64_t n, const double* A, double* F, double* J, double* H) {
   mf::GmfCoef bt;
-  mf::gmf_coefs(KIND, al, bt);
+  if (PA) mf::gmf_recips(bt);
+  else mf::gmf_coefs(KIND, al, bt);
   for (int64_t m = 0; m < n; m++) {
+    double nanf = 1.0;
+    if (PA) al = exponent_of(alv, m, nanf);
     mf::GmfOut o;
-    mf::gmf_lane<KIND, ORDER>(A + 9 * m, al, bt.b, o);
+    mf::gmf_lane<KIND, ORDER, PA>(A + 9 * m, al, bt.b, o);
+    for (int k = 0; k < 9; k++) o.F[k] *= nanf;
+    o.sc1 *= nanf;
+    o.sc2 *= nanf;
     for (int k = 0; k < 9; k++) F[9 * m + k] = o.F[k];
     if (ORDER < 1) continue;
     // the values the kernel stages: r1, r2, the distinct entries of the symmetric D2 / D3
@@ -77,10 +95,10 @@ void gmf_all(double al, int64_t n, const double* A, double* F, double* J, double
 static int smf_entry(bool mat, int kind, double al, int64_t n, const double* x, double* F, double* J, double* H) {
   if (kind != mf::POW) al = kind == mf::SQRT ? 0.5 : 0.0;
   switch (kind) {
-    case mf::EXP: smf_any<mf::EXP>(mat, al, n, x, F, J, H); break;
-    case mf::LOG: smf_any<mf::LOG>(mat, al, n, x, F, J, H); break;
-    case mf::SQRT: smf_any<mf::SQRT>(mat, al, n, x, F, J, H); break;
-    default: smf_any<mf::POW>(mat, al, n, x, F, J, H); break;
+    case mf::EXP: smf_any<mf::EXP>(mat, al, nullptr, n, x, F, J, H); break;
+    case mf::LOG: smf_any<mf::LOG>(mat, al, nullptr, n, x, F, J, H); break;
+    case mf::SQRT: smf_any<mf::SQRT>(mat, al, nullptr, n, x, F, J, H); break;
+    default: smf_any<mf::POW>(mat, al, nullptr, n, x, F, J, H); break;
   }
   return 0;
 }
@@ -91,11 +109,12 @@ extern "C" int lanecheck_smf_matrix(int kind, double al, int64_t n, const double
   return smf_entry(true, kind, al, n, A, F, J, H);
 }
 
-template <int KIND>
-static void gmf_order(int order, double al, int64_t n, const double* A, double* F, double* J, double* H) {
-  if (order == 2) gmf_all<KIND, 2>(al, n, A, F, J, H);
-  else if (order == 1) gmf_all<KIND, 1>(al, n, A, F, J, H);
-  else gmf_all<KIND, 0>(al, n, A, F, J, H);
+template <int KIND, bool PA = false>
+static void gmf_order(int order, double al, const double* alv, int64_t n, const double* A, double* F, double* J,
+                      double* H) {
+  if (order == 2) gmf_all<KIND, 2, PA>(al, alv, n, A, F, J, H);
+  else if (order == 1) gmf_all<KIND, 1, PA>(al, alv, n, A, F, J, H);
+  else gmf_all<KIND, 0, PA>(al, alv, n, A, F, J, H);
 }
 
 extern "C" int lanecheck_gmf(int kind, double al, int64_t n, const double* A, double* F, double* J, double* H) {
@@ -103,9 +122,24 @@ extern "C" int lanecheck_gmf(int kind, double al, int64_t n, const double* A, do
   if (kind != mf::POW) al = 0.0;
   const int order = H ? 2 : (J ? 1 : 0);
   switch (kind) {
-    case mf::EXP: gmf_order<mf::EXP>(order, al, n, A, F, J, H); break;
-    case mf::LOG: gmf_order<mf::LOG>(order, al, n, A, F, J, H); break;
-    default: gmf_order<mf::POW>(order, al, n, A, F, J, H); break;
+    case mf::EXP: gmf_order<mf::EXP>(order, al, nullptr, n, A, F, J, H); break;
+    case mf::LOG: gmf_order<mf::LOG>(order, al, nullptr, n, A, F, J, H); break;
+    default: gmf_order<mf::POW>(order, al, nullptr, n, A, F, J, H); break;
   }
+  return 0;
+}
+
+// powers with one exponent per matrix (host array alv)
+extern "C" int lanecheck_smf_pow(const double* alv, int64_t n, const double* v, double* F, double* J, double* H) {
+  smf_any<mf::POW, true>(false, 1.0, alv, n, v, F, J, H);
+  return 0;
+}
+extern "C" int lanecheck_smf_matrix_pow(const double* alv, int64_t n, const double* A, double* F, double* J,
+                                        double* H) {
+  smf_any<mf::POW, true>(true, 1.0, alv, n, A, F, J, H);
+  return 0;
+}
+extern "C" int lanecheck_gmf_pow(const double* alv, int64_t n, const double* A, double* F, double* J, double* H) {
+  gmf_order<mf::POW, true>(H ? 2 : (J ? 1 : 0), 1.0, alv, n, A, F, J, H);
   return 0;
 }

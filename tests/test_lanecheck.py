@@ -24,6 +24,8 @@ def lane():
     lib = ctypes.CDLL(LIB)
     for f in (lib.lanecheck_smf, lib.lanecheck_smf_matrix, lib.lanecheck_gmf):
         f.argtypes = [ctypes.c_int, ctypes.c_double, ctypes.c_int64] + [ctypes.c_void_p] * 4
+    for f in (lib.lanecheck_smf_pow, lib.lanecheck_smf_matrix_pow, lib.lanecheck_gmf_pow):
+        f.argtypes = [ctypes.c_void_p, ctypes.c_int64] + [ctypes.c_void_p] * 4
     return lib
 
 
@@ -74,6 +76,80 @@ def test_gmf_lane_math_vs_oracle(lane, kind, alpha):
         for name, x, y in zip("FJH", got[:order + 1], ref[:order + 1]):
             e = normwise(x, y)
             assert e.max() <= 1e-12, (kind, alpha, order, name, e.max(), int(np.argmax(e)))
+
+
+def run_pow(fn, alphas, X, nf, order):
+    X = np.ascontiguousarray(X, dtype=np.float64)
+    al = np.ascontiguousarray(alphas, dtype=np.float64)
+    n = len(X)
+    F = np.empty((n, nf))
+    J = np.empty((n, nf * nf)) if order >= 1 else None
+    H = np.empty((n, nf ** 3)) if order >= 2 else None
+    p = lambda a: None if a is None else a.ctypes.data
+    fn(al.ctypes.data, n, X.ctypes.data, F.ctypes.data, p(J), p(H))
+    return F, J, H
+
+
+def mixed_alphas(n, seed):
+    """exponents uniform in [-1, 1] with the special values 0, +-1, 1/2 and a few large ones (Taylor radius 3/|al|)"""
+    al = np.random.default_rng(seed).uniform(-1.0, 1.0, n)
+    al[:8] = [0.0, 1.0, -1.0, 0.5, -0.5, 2.0, 40.0, -40.0]
+    return al
+
+
+def test_smf_per_matrix_exponents_vs_oracle(lane):
+    """per-lane series coefficients (PA) against the oracle's per-matrix exponents, matrix layout, order 2"""
+    o = Checker("oracle")
+    A = W.spd(3000, seed=34)
+    al = mixed_alphas(len(A), 35)
+    al[40:] = np.where(np.abs(al[40:]) < 1e-3, 1e-3, al[40:])
+    got = run_pow(lane.lanecheck_smf_matrix_pow, al, A, 9, 2)
+    _, *ref = o.smf_matrix_alpha(al, A, 2)
+    for name, x, y in zip("FJH", got, ref):
+        e = normwise(x, y)
+        assert e.max() <= 1e-12, (name, e.max(), int(np.argmax(e)), al[int(np.argmax(e))])
+
+
+def test_smf_voigt_per_matrix_exponents_equal_uniform_calls(lane):
+    """Voigt layout: each matrix's result with its own exponent equals the uniform-exponent path at that exponent
+    (the two differ only in how the series coefficients binom(al, k) are rounded)"""
+    v = W.voigt(W.spd(400, seed=36))
+    al = np.repeat(np.array([0.3, -0.7, 0.0, 1.0, 2.5, 40.0, -1.0, 0.9]), 50)
+    got = run_pow(lane.lanecheck_smf_pow, al, v, 6, 2)
+    for a in np.unique(al):
+        sel = al == a
+        ref = run(lane.lanecheck_smf, "pow", a, v[sel], 6, 2)
+        for name, x, y in zip("FJH", got, ref):
+            e = normwise(x[sel], y)
+            assert e.max() <= 1e-13, (a, name, e.max())
+
+
+def test_gmf_per_matrix_exponents_vs_oracle(lane):
+    o = Checker("oracle")
+    G = W.general(3000, seed=37)
+    al = mixed_alphas(len(G), 38)
+    for order in (0, 1, 2):
+        got = run_pow(lane.lanecheck_gmf_pow, al, G, 9, order)
+        _, *ref = o.gmf_alpha(al, G, order)
+        for name, x, y in zip("FJH", got[:order + 1], ref[:order + 1]):
+            e = normwise(x, y)
+            assert e.max() <= 1e-12, (order, name, e.max(), int(np.argmax(e)), al[int(np.argmax(e))])
+
+
+def test_non_finite_exponent_gives_nan_for_its_matrix_only(lane):
+    A = W.spd(64, seed=39)
+    G = W.general(64, seed=40)
+    al = np.full(64, 0.4)
+    al[[3, 17, 50]] = [np.nan, np.inf, -np.inf]
+    bad = np.zeros(64, bool)
+    bad[[3, 17, 50]] = True
+    for fn, X in ((lane.lanecheck_smf_matrix_pow, A), (lane.lanecheck_gmf_pow, G)):
+        F, J, H = run_pow(fn, al, X, 9, 2)
+        for x in (F, J, H):
+            assert np.isnan(x[bad]).all()
+        ref = run_pow(fn, np.full(64, 0.4), X, 9, 2)
+        for x, y in zip((F, J, H), ref):
+            assert np.array_equal(x[~bad], y[~bad], equal_nan=True)
 
 
 def test_golden_vectors(lane):

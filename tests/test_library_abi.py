@@ -22,7 +22,8 @@ def declared():
 
 def test_header_declares_the_entry_points():
     names = declared()
-    for n in ("smf_hip_eval", "smf_hip_eval_matrix", "gmf_hip_eval", "smf_hip_version", "smf_hip_target"):
+    for n in ("smf_hip_eval", "smf_hip_eval_matrix", "gmf_hip_eval", "smf_hip_version", "smf_hip_target",
+              "smf_hip_eval_pow", "smf_hip_eval_matrix_pow", "gmf_hip_eval_pow"):
         assert n in names
 
 
@@ -59,3 +60,20 @@ def test_invalid_arguments_rejected_before_any_device_work():
     assert lib.smf_hip_eval(0, 0.0, (2 ** 31 - 1) * 32 + 1, dummy, dummy, None, None, None) == -2
     assert lib.smf_hip_eval_matrix(0, 0.0, 2 ** 40, dummy, dummy, None, None, None) == -2
     assert lib.gmf_hip_eval(0, 0.0, (2 ** 31 - 1) * 64 + 1, dummy, dummy, None, None, None) == -2
+
+
+@pytest.mark.skipif(not os.path.exists(LIB), reason="HIP library not built (python -m matfun_amd.build)")
+def test_per_matrix_exponent_entry_points_reject_before_device_work():
+    lib = ctypes.CDLL(LIB)
+    args = [ctypes.c_void_p, ctypes.c_int64] + [ctypes.c_void_p] * 5
+    fns = (lib.smf_hip_eval_pow, lib.smf_hip_eval_matrix_pow, lib.gmf_hip_eval_pow, lib.smf_hip_eval_pow_mixed,
+           lib.smf_hip_eval_matrix_pow_mixed, lib.gmf_hip_eval_pow_mixed)
+    dummy = ctypes.c_void_p(0x1000)
+    for f in fns:
+        f.argtypes, f.restype = args, ctypes.c_int
+        assert f(dummy, -1, dummy, dummy, None, None, None) == -1          # n < 0
+        assert f(dummy, 10, dummy, dummy, None, dummy, None) == -1         # H without J
+        assert f(None, 10, dummy, dummy, None, None, None) == -1           # no exponents
+        assert f(None, 0, None, None, None, None, None) == 0               # n == 0: nothing to do
+    assert lib.smf_hip_eval_pow(dummy, 2 ** 40, dummy, dummy, None, None, None) == -2
+    assert lib.gmf_hip_eval_pow(dummy, (2 ** 31 - 1) * 64 + 1, dummy, dummy, None, None, None) == -2

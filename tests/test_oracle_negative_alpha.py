@@ -111,3 +111,41 @@ def test_alpha_zero_is_identity_with_zero_derivatives(oracle):
     # gmf assembles J and H from residue sums that cancel to zero only up to rounding (eigenvalue separations of the
     # random general matrices reach 1e-2, amplifying it by their inverse powers)
     assert np.abs(Jg[ok]).max() <= 1e-11 and np.abs(Hg[ok]).max() <= 1e-9
+
+
+def voigt_to_matrix(Fv, Jv, Hv):
+    """Voigt results -> matrix layout: entry (a, b) reads Voigt component k(a, b); derivative directions E_cd with
+    c != d carry weight 1/2 (each Voigt off-diagonal component moves A_cd and A_dc)"""
+    vm = np.array([0, 3, 4, 3, 1, 5, 4, 5, 2])
+    w = np.array([1.0 if m % 4 == 0 else 0.5 for m in range(9)])
+    n = len(Fv)
+    F = Fv[:, vm]
+    J = Jv.reshape(n, 6, 6)[:, vm][:, :, vm] * w[None, None, :]
+    H = Hv.reshape(n, 6, 6, 6)[:, vm][:, :, vm][:, :, :, vm] * (w[:, None] * w[None, :])[None, None]
+    return F, J.reshape(n, 81), H.reshape(n, 729)
+
+
+def test_smf_voigt_alpha_equals_matrix_alpha(oracle):
+    """the Voigt-layout per-matrix exponents give the matrix-layout results (pinned above) after the layout map"""
+    A = W.spd(200, seed=47)
+    al = np.random.default_rng(48).uniform(-1.0, 1.0, len(A))
+    al[:3] = [0.0, -1.0, 1.0]
+    v = A.reshape(-1, 3, 3)[:, [0, 1, 2, 0, 0, 1], [0, 1, 2, 1, 2, 2]]
+    rc, Fv, Jv, Hv = oracle.smf_alpha(al, v, 2)
+    assert rc == 0
+    _, F, J, H = oracle.smf_matrix_alpha(al, A, 2)
+    for x, y in zip(voigt_to_matrix(Fv, Jv, Hv), (F, J, H)):
+        assert np.array_equal(x, y)
+    # and for exponents > 0, the reference semantics: one uniform call per exponent
+    for a in (0.3, 2.5):
+        _, Fu, Ju, Hu = oracle.smf("pow", a, v, 2)
+        _, Fa, Ja, Ha = oracle.smf_alpha(np.full(len(v), a), v, 2)
+        assert np.array_equal(Fu, Fa) and np.array_equal(Ju, Ja) and np.array_equal(Hu, Ha)
+
+
+def test_non_finite_exponent_rejected(oracle):
+    A = W.spd(4, seed=49)
+    for bad in (np.nan, np.inf):
+        al = np.array([0.5, bad, 0.5, 0.5])
+        assert oracle.smf_matrix_alpha(al, A, 1)[0] == -1
+        assert oracle.gmf_alpha(al, A, 1)[0] == -1
