@@ -8,8 +8,12 @@ matrices (one smf_hip_eval_matrix call each), inputs resident in HBM, outputs ov
 Also measured (same JSON line, key "legs"): the mixed form of the same workload (FP32 outputs; log, sqrt, pow(0.3):
 exp of eigenvalues up to 1e2 is beyond the FP32 range) with its error against the FP64 form, and general matrices
 (gmf_hip_eval, n = 2^20 per function, similarity / complex-pair / shear / 4I+N mix, seed 0) in FP64 and mixed form.
+Key "mixed_vs_fp64": the mixed form against the FP64 form on smf log and smf pow with one exponent per matrix drawn
+uniform in [-1, 1] (order 2, n = 2^24, the headline distribution, seed 1) plus gmf log order 1 (n = 2^22, the general
+mix, seed 1): time per step of each form, the speed-up, and the mixed form's error against the FP64 form per output.
 
     python bench.py [--steps K] [--warmup W] [--n N] [--gmf-n N] [--gpus 1] [--no-cpu] [--only-headline]
+                    [--mixed-n N] [--mixed-gmf-n N]
 
 `value` = matrices/s of the headline step (4 n matrices per step).  `roofline`: the dominant kernel (the smf order-2
 matrix-layout FP64 kernel): algorithmic bytes per launch (A in + F, J, H out: 6624 B per matrix) over its average
@@ -81,6 +85,97 @@ def cpu_baseline(A, seconds=10.0, chunk=1 << 17):
                       f"order 2 matrix layout, {dt:.1f} s wall"}
 
 
+def timed(torch, stream, calls, steps, warmup):
+    """run the closures `calls` (one step) warmup + steps times; average kernel time of each (HIP events on the
+    launch stream) and the wall time per step"""
+    ev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in calls]
+    for _ in range(warmup):
+        for c in calls:
+            assert c() == 0
+    torch.cuda.synchronize()
+    ks = [[] for _ in calls]
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        for i, c in enumerate(calls):
+            ev[i][0].record(stream)
+            assert c() == 0
+            ev[i][1].record(stream)
+        torch.cuda.synchronize()
+        for i, (a, b) in enumerate(ev):
+            ks[i].append(a.elapsed_time(b))
+    return [float(np.mean(k)) for k in ks], (time.perf_counter() - t0) / steps
+
+
+def mixed_vs_fp64(torch, lib, W, dev, stream, args, sample=1 << 16):
+    """smf log + smf pow (per-matrix exponent ~ U[-1, 1]) order 2 and gmf log order 1, FP64 form vs mixed form"""
+    import matfun_amd
+    n, ng = args.mixed_n, args.mixed_gmf_n
+    log(f"bench: mixed vs fp64: generating {n} symmetric and {ng} general inputs (seed 1)")
+    dA = torch.from_numpy(W.spd(n, seed=1)).to(dev)
+    dal = torch.from_numpy(np.random.default_rng(1).uniform(-1.0, 1.0, n)).to(dev)
+    dG = torch.from_numpy(W.general(ng, seed=1)).to(dev)
+    sh = stream.cuda_stream
+    res = {"workload": f"smf log + smf pow(alpha per matrix ~ U[-1,1]), matrix layout, order 2, n={n} each; gmf log "
+                       f"order 1, n={ng}", "n_smf": n, "n_gmf": ng}
+    t = {}
+    for form, dt, eb in (("fp64", torch.float64, 8), ("mixed", torch.float32, 4)):
+        mixed = form == "mixed"
+        F = torch.empty(n * 9, dtype=dt, device=dev)
+        J = torch.empty(n * 81, dtype=dt, device=dev)
+        H = torch.empty(n * 729, dtype=dt, device=dev)
+        p = (F.data_ptr(), J.data_ptr(), H.data_ptr())
+        smf = lib.smf_hip_eval_matrix_mixed if mixed else lib.smf_hip_eval_matrix
+        smfp = lib.smf_hip_eval_matrix_pow_mixed if mixed else lib.smf_hip_eval_matrix_pow
+        gmf = lib.gmf_hip_eval_mixed if mixed else lib.gmf_hip_eval
+        calls = [lambda: smf(matfun_amd.KINDS["log"], 0.0, n, dA.data_ptr(), *p, sh),
+                 lambda: smfp(dal.data_ptr(), n, dA.data_ptr(), *p, sh),
+                 lambda: gmf(matfun_amd.KINDS["log"], 0.0, ng, dG.data_ptr(), p[0], p[1], None, sh)]
+        ks, wall = timed(torch, stream, calls, args.steps, args.warmup)
+        b2, b1 = bytes_per_matrix(2, eb), bytes_per_matrix(1, eb)
+        res[form] = {"ms_per_step": wall * 1e3, "avg_kernel_ms": {"smf_log": ks[0], "smf_pow_alpha": ks[1],
+                                                                  "gmf_log_o1": ks[2]},
+                     "frac_hbm": {"smf_log": n * b2 / (ks[0] * 1e-3) / 1e9 / HBM_PEAK_GBS,
+                                  "smf_pow_alpha": n * b2 / (ks[1] * 1e-3) / 1e9 / HBM_PEAK_GBS,
+                                  "gmf_log_o1": ng * b1 / (ks[2] * 1e-3) / 1e9 / HBM_PEAK_GBS},
+                     "matrices_per_s": (2 * n + ng) / (sum(ks) * 1e-3)}
+        t[form] = sum(ks)
+        log(f"bench: mixed vs fp64: {form}: kernels {ks[0]:.2f} + {ks[1]:.2f} + {ks[2]:.2f} ms")
+        del F, J, H
+        torch.cuda.empty_cache()
+    res["speedup_mixed_vs_fp64"] = t["fp64"] / t["mixed"]
+    # error of the mixed form against the FP64 form (itself checked against the CPU oracle by the test suite) on the
+    # first `sample` matrices of each call
+    err = {}
+    m = min(sample, n)
+    for name, order, run in (
+            ("smf_log", 2, lambda mx, o: (lib.smf_hip_eval_matrix_mixed if mx else lib.smf_hip_eval_matrix)(
+                matfun_amd.KINDS["log"], 0.0, m, dA.data_ptr(), *o, sh)),
+            ("smf_pow_alpha", 2, lambda mx, o: (lib.smf_hip_eval_matrix_pow_mixed if mx else
+                                                lib.smf_hip_eval_matrix_pow)(dal.data_ptr(), m, dA.data_ptr(), *o, sh)),
+            ("gmf_log_o1", 1, lambda mx, o: (lib.gmf_hip_eval_mixed if mx else lib.gmf_hip_eval)(
+                matfun_amd.KINDS["log"], 0.0, min(m, ng), dG.data_ptr(), o[0], o[1], None, sh))):
+        mm = min(m, ng) if name.startswith("gmf") else m
+        outs = []
+        for mx, dt in ((False, torch.float64), (True, torch.float32)):
+            o = [torch.empty((mm, s), dtype=dt, device=dev) for s in (9, 81, 729)]
+            assert run(mx, [x.data_ptr() for x in o]) == 0
+            outs.append(o)
+        torch.cuda.synchronize()
+        valid = ~torch.isnan(outs[0][0]).any(dim=1)
+        e = {"sample": mm, "nan_matrices": int((~valid).sum())}
+        for q, nm in enumerate("FJH"[:order + 1]):
+            a, b = outs[0][q], outs[1][q]
+            r = (torch.linalg.norm(b.double() - a, dim=1) / torch.linalg.norm(a, dim=1))[valid]
+            e[nm] = {"max": float(r.max()), "p99.9": float(torch.quantile(r[: 1 << 24], 0.999)),
+                     "median": float(r.median())}
+        err[name] = e
+    res["mixed_err_vs_fp64"] = err
+    log(f"bench: mixed vs fp64: speed-up {res['speedup_mixed_vs_fp64']:.3f}")
+    del dA, dal, dG
+    torch.cuda.empty_cache()
+    return res
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -91,6 +186,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--only-headline", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
+    ap.add_argument("--mixed-n", type=int, default=1 << 24)
+    ap.add_argument("--mixed-gmf-n", type=int, default=1 << 22)
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU only; --gpus must be 1 (the batch shards trivially "
@@ -191,6 +288,10 @@ def main():
         legs["gmf_mixed"]["speedup_vs_fp64"] = legs["gmf_mixed"]["matrices_per_s"] / legs["gmf_fp64"]["matrices_per_s"]
         legs["mixed_vs_fp64_smf_plus_gmf"] = t64 / tmx
 
+    if not args.only_headline:
+        del Fb, Jb, Hb
+        torch.cuda.empty_cache()
+        legs_c = mixed_vs_fp64(torch, lib, W, dev, stream, args)
     # device copy bandwidth (read + write of 2 GiB), for context next to the 8 TB/s spec
     x = torch.empty(1 << 28, dtype=torch.float64, device=dev)
     y = torch.empty_like(x)
@@ -216,6 +317,8 @@ def main():
                         "frac": head["frac"], "traffic": profiled_traffic(n), "kernel": "mf::smf_kernel<KIND,true,2,double>",
                         "avg_kernel_ms": head["avg_kernel_ms"], "measured_copy_GBs": copy_gbs},
            "legs": legs}
+    if not args.only_headline:
+        out["mixed_vs_fp64"] = legs_c
     if not args.no_cpu:
         log("bench: CPU baseline")
         out["cpu_baseline"] = cpu_baseline(A, seconds=args.cpu_seconds)
