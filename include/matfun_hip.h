@@ -68,6 +68,68 @@ int smf_hip_eval_matrix_pow_mixed(const double *alpha, int64_t n, const double *
 int gmf_hip_eval_pow_mixed(const double *alpha, int64_t n, const double *A, float *F, float *J, float *H,
                            void *stream);
 
+/* ---- heterogeneous batches (no counterpart in the reference API, which evaluates one function per call): one
+ * persistent kernel evaluates groups of matrices with different functions and orders -- e.g. the integration points
+ * of finite elements, each element with its own constitutive function.  Groups follow one another: the matrices of
+ * group g are rows first_g .. first_g + count_g - 1 of A (N x 9, 3x3 matrix layout for both families) and F (N x 9);
+ * J has one row of 81 per matrix of order >= 1 and H one row of 729 per matrix of order 2, in the same order (rows
+ * of matrices of lower order are skipped, not left empty).  smf groups take symmetric matrices (NaN results for a
+ * non-symmetric one, as smf_hip_eval_matrix).  Results are those of the homogeneous entry points on the same
+ * matrices: smf_hip_eval_matrix / gmf_hip_eval, and for pow smf_hip_eval_matrix_pow / gmf_hip_eval_pow with the
+ * per-matrix exponents (bitwise: the same tile code runs). */
+typedef struct mf_hip_group {
+  int32_t family;   /* 0 smf (symmetric), 1 gmf (general) */
+  int32_t kind;     /* 0 exp, 1 log, 2 pow, 3 sqrt */
+  int32_t order;    /* 0: F; 1: F, J; 2: F, J, H */
+  int32_t count;    /* matrices in the group, >= 0 */
+  double alpha;     /* pow: the group's exponent (finite) when mf_hip_batch_eval gets no per-matrix exponents */
+} mf_hip_group;
+
+typedef struct mf_hip_plan mf_hip_plan;          /* opaque */
+typedef struct mf_hip_sequence mf_hip_sequence;  /* opaque */
+
+/* Host set-up (allocates device tables: tiles of up to 64 matrices of one class, per-matrix rows, series tables;
+ * synchronous).  Returns 0, -1 invalid groups, -2 device error. */
+int mf_hip_plan_create(const mf_hip_group *groups, int64_t ngroups, mf_hip_plan **plan);
+int mf_hip_plan_destroy(mf_hip_plan *plan);
+/* sizes of the batch: what 0 = matrices N, 1 = J rows, 2 = H rows, 3 = tiles, 4 = classes present */
+int64_t mf_hip_plan_size(const mf_hip_plan *plan, int what);
+
+/* Evaluate the batch (asynchronous on `stream`, allocates nothing, capturable in a hipGraph).  alpha: NULL (each pow
+ * group uses its alpha) or a device array of N exponents (read for the matrices of pow groups only; any finite value,
+ * NaN results for a non-finite one).  J / H must be non-NULL when a group of order >= 1 / 2 is present.  Returns 0,
+ * -1 invalid arguments, -2 launch error. */
+int mf_hip_batch_eval(const mf_hip_plan *plan, const double *A, const double *alpha, double *F, double *J, double *H,
+                      void *stream);
+int mf_hip_batch_eval_mixed(const mf_hip_plan *plan, const double *A, const double *alpha, float *F, float *J,
+                            float *H, void *stream);
+
+/* Launch strategy (every strategy gives bitwise the same results):
+ *   mode 0 fused: one kernel over the tiles of all classes, tiles in class order (order 0) or interleaved in
+ *          proportion to the classes' sizes (order 1)
+ *   mode 1 split: one kernel per class (each compiled for its class alone)
+ *   per_cu > 0: persistent, per_cu workgroups per compute unit loop over the tiles; 0: one workgroup per tile
+ * mf_hip_plan_tune times the candidates on the caller's buffers (the batch is evaluated several times) and keeps the
+ * fastest; `result` (optional) receives {mode, per_cu, order, microseconds of the best}.  Synchronous. */
+int mf_hip_plan_set_strategy(mf_hip_plan *plan, int mode, int per_cu, int order);
+int mf_hip_plan_tune(mf_hip_plan *plan, const double *A, const double *alpha, void *F, void *J, void *H, int mixed,
+                     void *stream, double *result);
+
+/* Newton-iteration driver: A <- A o (1 + rel_sigma Z), Z standard normal from Philox4x32-10 (Salmon et al., SC'11)
+ * keyed by seed, counter (matrix, iteration, block): entry-wise relative noise, the same for (i, j) and (j, i) of
+ * smf matrices so that they stay symmetric.  Asynchronous on `stream`. */
+int mf_hip_perturb(const mf_hip_plan *plan, double *A, double rel_sigma, uint64_t seed, int iteration, void *stream);
+
+/* A sequence of Newton iterations captured in one hipGraph: A <- A0 (device copy), then `iterations` times
+ * { mf_hip_perturb(iteration i = 1..iterations); mf_hip_batch_eval(_mixed) } with the plan's strategy at creation.
+ * F / J / H hold the last iteration's results.  Replays are independent (each starts from A0).  Creation captures and
+ * instantiates (synchronous); launch is asynchronous on `stream`. */
+int mf_hip_sequence_create(const mf_hip_plan *plan, const double *A0, double *A, const double *alpha, void *F, void *J,
+                           void *H, int mixed, int iterations, double rel_sigma, uint64_t seed,
+                           mf_hip_sequence **seq);
+int mf_hip_sequence_launch(mf_hip_sequence *seq, void *stream);
+int mf_hip_sequence_destroy(mf_hip_sequence *seq);
+
 /* library version ("0.1.3"-compatible API level plus this build); replaces smf_version (c/include/matfun/matfun.h:53) */
 const char *smf_hip_version(void);
 

@@ -19,7 +19,7 @@ import math
 import os
 
 __version__ = "0.1.3"
-__all__ = ["smf", "gmf", "library", "library_path", "backends", "KINDS"]
+__all__ = ["smf", "gmf", "Batch", "Group", "library", "library_path", "backends", "KINDS"]
 
 KINDS = {"exp": 0, "log": 1, "pow": 2, "sqrt": 3}
 _HERE = os.path.dirname(os.path.abspath(__file__))
@@ -148,3 +148,147 @@ def gmf(kind, A, alpha=None, order=0, precision="fp64"):
     fn = ((lib.gmf_hip_eval_pow_mixed if mixed else lib.gmf_hip_eval_pow) if pa else
           (lib.gmf_hip_eval_mixed if mixed else lib.gmf_hip_eval))
     return _run(fn, kind, alpha, A, shape[:-2], ((3, 3), (3, 3, 3, 3), (3, 3, 3, 3, 3, 3)), order, dt, pa)
+
+
+# ---- heterogeneous batches (include/matfun_hip.h, mf_hip_*)
+class Group(ctypes.Structure):
+    """one group of matrices with a common function: family 0 smf / 1 gmf, kind, order, count, alpha (pow)"""
+    _fields_ = [("family", ctypes.c_int32), ("kind", ctypes.c_int32), ("order", ctypes.c_int32),
+                ("count", ctypes.c_int32), ("alpha", ctypes.c_double)]
+
+
+def _batch_lib():
+    lib = library()
+    if not getattr(lib, "_batch_ready", False):
+        P = ctypes.c_void_p
+        lib.mf_hip_plan_create.argtypes = [ctypes.POINTER(Group), ctypes.c_int64, ctypes.POINTER(P)]
+        lib.mf_hip_plan_destroy.argtypes = [P]
+        lib.mf_hip_plan_size.argtypes, lib.mf_hip_plan_size.restype = [P, ctypes.c_int], ctypes.c_int64
+        for fn in (lib.mf_hip_batch_eval, lib.mf_hip_batch_eval_mixed):
+            fn.argtypes = [P] * 7
+        lib.mf_hip_plan_set_strategy.argtypes = [P, ctypes.c_int, ctypes.c_int, ctypes.c_int]
+        lib.mf_hip_plan_tune.argtypes = [P] * 6 + [ctypes.c_int, P, ctypes.POINTER(ctypes.c_double)]
+        lib.mf_hip_perturb.argtypes = [P, P, ctypes.c_double, ctypes.c_uint64, ctypes.c_int, P]
+        lib.mf_hip_sequence_create.argtypes = [P] * 7 + [ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_uint64,
+                                                         ctypes.POINTER(P)]
+        lib.mf_hip_sequence_launch.argtypes = [P, P]
+        lib.mf_hip_sequence_destroy.argtypes = [P]
+        lib._batch_ready = True
+    return lib
+
+
+FAMILIES = {"smf": 0, "gmf": 1}
+
+
+class Batch:
+    """A heterogeneous batch: groups (family, kind, order, count[, alpha]) of 3x3 matrices, evaluated by one kernel.
+
+        b = matfun_amd.Batch([("smf", "log", 2, 27), ("gmf", "exp", 1, 8), ("smf", "pow", 1, 4, 0.5), ...])
+        F, J, H = b.eval(A)                 A: (N, 3, 3) float64 CUDA tensor, groups one after another
+        F, J, H = b.eval(A, alpha=al)       al: (N,) per-matrix exponents (read for pow groups)
+
+    J holds a row (3, 3, 3, 3) per matrix of order >= 1 and H a row (3,)*6 per matrix of order 2, in group order.
+    Device tensors in, device tensors out (torch's current stream)."""
+
+    def __init__(self, groups):
+        lib = _batch_lib()
+        arr = (Group * max(1, len(groups)))()
+        for i, g in enumerate(groups):
+            fam, kind, order, count = g[:4]
+            alpha = g[4] if len(g) > 4 else 0.0
+            arr[i] = Group(FAMILIES.get(fam, fam) if isinstance(fam, str) else int(fam),
+                           KINDS[kind] if isinstance(kind, str) else int(kind), int(order), int(count), float(alpha))
+        h = ctypes.c_void_p()
+        r = lib.mf_hip_plan_create(arr, len(groups), ctypes.byref(h))
+        if r != 0:
+            raise ValueError(f"mf_hip_plan_create returned {r}")
+        self._h, self._lib = h, lib
+        self.n, self.nj, self.nh, self.ntiles, self.nclasses = (int(lib.mf_hip_plan_size(h, w)) for w in range(5))
+        self.groups = list(groups)
+
+    def close(self):
+        if self._h:
+            self._lib.mf_hip_plan_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def outputs(self, precision="fp64"):
+        import torch
+        dt = torch.float32 if precision == "mixed" else torch.float64
+        return (torch.empty((self.n, 3, 3), dtype=dt, device="cuda"),
+                torch.empty((self.nj,) + (3,) * 4, dtype=dt, device="cuda"),
+                torch.empty((self.nh,) + (3,) * 6, dtype=dt, device="cuda"))
+
+    @staticmethod
+    def _p(t):
+        return None if t is None or t.numel() == 0 else t.data_ptr()
+
+    def eval(self, A, alpha=None, out=None, precision="fp64"):
+        import torch
+        F, J, H = out if out is not None else self.outputs(precision)
+        fn = self._lib.mf_hip_batch_eval_mixed if precision == "mixed" else self._lib.mf_hip_batch_eval
+        r = fn(self._h, self._p(A), self._p(alpha), self._p(F), self._p(J), self._p(H),
+               torch.cuda.current_stream().cuda_stream)
+        if r != 0:
+            raise ValueError(f"mf_hip_batch_eval returned {r}")
+        return F, J, H
+
+    def set_strategy(self, mode, per_cu, order=1):
+        if self._lib.mf_hip_plan_set_strategy(self._h, int(mode), int(per_cu), int(order)) != 0:
+            raise ValueError("invalid strategy")
+
+    def tune(self, A, alpha=None, out=None, precision="fp64"):
+        import torch
+        F, J, H = out if out is not None else self.outputs(precision)
+        res = (ctypes.c_double * 4)()
+        r = self._lib.mf_hip_plan_tune(self._h, self._p(A), self._p(alpha), self._p(F), self._p(J), self._p(H),
+                                       1 if precision == "mixed" else 0, torch.cuda.current_stream().cuda_stream, res)
+        if r != 0:
+            raise ValueError(f"mf_hip_plan_tune returned {r}")
+        return {"mode": int(res[0]), "per_cu": int(res[1]), "order": int(res[2]), "us": res[3]}
+
+    def perturb(self, A, sigma, seed, iteration):
+        import torch
+        r = self._lib.mf_hip_perturb(self._h, self._p(A), float(sigma), int(seed), int(iteration),
+                                     torch.cuda.current_stream().cuda_stream)
+        if r != 0:
+            raise ValueError(f"mf_hip_perturb returned {r}")
+
+    def sequence(self, A0, A, out, iterations, sigma, seed, alpha=None, precision="fp64"):
+        """a graph-captured Newton sequence (mf_hip_sequence_*): .launch() replays it on the current stream"""
+        return Sequence(self, A0, A, out, iterations, sigma, seed, alpha, precision)
+
+
+class Sequence:
+    def __init__(self, batch, A0, A, out, iterations, sigma, seed, alpha, precision):
+        F, J, H = out
+        h = ctypes.c_void_p()
+        p = Batch._p
+        r = batch._lib.mf_hip_sequence_create(batch._h, p(A0), p(A), p(alpha), p(F), p(J), p(H),
+                                              1 if precision == "mixed" else 0, int(iterations), float(sigma),
+                                              int(seed), ctypes.byref(h))
+        if r != 0:
+            raise ValueError(f"mf_hip_sequence_create returned {r}")
+        self._h, self._lib, self._keep = h, batch._lib, (batch, A0, A, alpha, out)
+
+    def launch(self):
+        import torch
+        r = self._lib.mf_hip_sequence_launch(self._h, torch.cuda.current_stream().cuda_stream)
+        if r != 0:
+            raise ValueError(f"mf_hip_sequence_launch returned {r}")
+
+    def close(self):
+        if self._h:
+            self._lib.mf_hip_sequence_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

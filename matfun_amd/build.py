@@ -13,7 +13,12 @@ import sys
 HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libmatfun_hip.so")
-SOURCES = ["matfun_hip.hip", "smf_kernels.hip", "gmf_kernels.hip"]
+SOURCES = ["matfun_hip.hip", "smf_kernels.hip", "gmf_kernels.hip", "batch.hip", "batch_fused.hip", "batch_smf.hip",
+           "batch_gmf.hip"]
+# the heterogeneous-batch kernels: MachineLICM off, so that no per-tile constant is hoisted out of the persistent loop
+# over tiles and held in registers across it (batch_kernels.hpp)
+PER_FILE = {"batch_fused.hip": ["-mllvm", "-disable-machine-licm"], "batch_smf.hip": ["-mllvm", "-disable-machine-licm"],
+            "batch_gmf.hip": ["-mllvm", "-disable-machine-licm"]}
 ARCH = os.environ.get("MATFUN_HIP_ARCH", "gfx950")
 
 
@@ -30,14 +35,20 @@ def flags(extra=()):
 
 
 def build(verbose=False, extra=()):
-    objs = []
+    """compile the sources in parallel (the batch kernels take minutes each), link libmatfun_hip.so"""
+    from concurrent.futures import ThreadPoolExecutor
+    objs, cmds = [], []
     for src in SOURCES:
         obj = os.path.join(CSRC, os.path.splitext(src)[0] + ".o")
-        cmd = [hipcc(), *flags(extra), "-c", os.path.join(CSRC, src), "-o", obj]
-        if verbose:
-            print(" ".join(cmd), flush=True)
-        subprocess.run(cmd, check=True)
+        cmds.append([hipcc(), *flags(list(extra) + PER_FILE.get(src, [])), "-c", os.path.join(CSRC, src), "-o", obj])
         objs.append(obj)
+        if verbose:
+            print(" ".join(cmds[-1]), flush=True)
+    with ThreadPoolExecutor(max_workers=min(len(cmds), os.cpu_count() or 1)) as ex:
+        for r in list(ex.map(lambda c: subprocess.run(c, capture_output=True, text=True), cmds)):
+            if r.returncode != 0:
+                sys.stderr.write(r.stderr)
+                raise subprocess.CalledProcessError(r.returncode, r.args)
     tmp = LIB + ".tmp"
     cmd = [hipcc(), "--offload-arch=" + ARCH, "-shared", "-fPIC", *objs, "-o", tmp]
     subprocess.run(cmd, check=True)

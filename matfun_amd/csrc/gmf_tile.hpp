@@ -1,0 +1,148 @@
+// gmf_tile.hpp -- one tile (up to 64 general real 3x3 matrices, one wavefront) of the general kernels:
+//   phase 1, one matrix per lane: eigenvalues, divided differences, real basis, contracted tensors (gmf_device.hpp);
+//            r_m (27 values) and D3 (27) staged in LDS per matrix, D2 (9) and F (9) in a second region
+//   phase 2, lane groups: 27 lanes per matrix, two matrices per pass, assemble J (81: 3 per lane) and H (729: 27 per
+//            lane) into an LDS tile in the API layout, which the wave then writes as one contiguous block (2 x 5.8 KB
+//            of consecutive output per pass; whole cache lines per store instruction)
+// OT = double: the FP64 form; OT = float: the mixed form (FP64 per-lane math, FP32 staged tensors, J / H assembly
+// and outputs).  PA: powers with one exponent per matrix (alv[i], read by its lane; bt then holds 1 / j; a matrix
+// whose exponent is not finite gets NaN results).  Used by the homogeneous kernels (gmf_kernels.hip, RunRows) and the
+// heterogeneous batch kernel (batch.hip, ListRows).
+#pragma once
+#include "gmf_device.hpp"
+#include "tile_rows.hpp"
+
+namespace mf {
+
+// LDS layout (elements of OT):
+//   main [64][MS]   r1 (9), r2 (9); order 2: the 10 distinct D3 and sc2 (MS = 29; order 1: 19)
+//   x    [XS]       phase 1: F of the 64 matrices [0, 576), then D2 (6 distinct) and sc1 per matrix [162, 610);
+//                   J passes: the 2 x 81 output tile [0, 162); H passes: a half-block tile 2 x 378 (or 2 x 351)
+// (the FP64 input block of 64 x 9 doubles overlays the start, before phase 1)
+template <int ORDER, class OT>
+struct GmfLds {
+  static constexpr int MS = ORDER >= 2 ? 29 : 19;
+  static constexpr int XAUX = 162;
+  static constexpr int XS = ORDER == 0 ? 576 : (ORDER == 1 ? 610 : 756);
+  static constexpr int MAIN = ORDER == 0 ? 0 : 64 * MS;
+  static constexpr int TOT = MAIN + XS;
+  static constexpr int BYTES = (TOT * (int)sizeof(OT) > WAVE * 9 * 8) ? TOT * (int)sizeof(OT) : WAVE * 9 * 8;
+};
+
+// one tile of nm <= 64 matrices; lds: GmfLds<ORDER, OT>::BYTES bytes, 16-byte aligned; lane = threadIdx.x.  Ends
+// with the tile's outputs written and a barrier (the LDS may be reused).
+template <int KIND, int ORDER, class OT, bool PA, class R>
+MF_DEV void gmf_tile(const R& rows, int nm, const double* __restrict__ A, OT* __restrict__ F, OT* __restrict__ J,
+                     OT* __restrict__ H, double al, const double* __restrict__ alv, const double* __restrict__ bt,
+                     unsigned char* lds, int lane) {
+  using L = GmfLds<ORDER, OT>;
+  double* xin = reinterpret_cast<double*>(lds);
+  OT* mainr = reinterpret_cast<OT*>(lds);
+  OT* xr = mainr + L::MAIN;
+
+  load_in<9>(xin, A, rows, nm, lane);
+  __syncthreads();
+  double a[9];
+#pragma unroll
+  for (int k = 0; k < 9; k++) a[k] = lane < nm ? xin[9 * lane + k] : (k % 4 == 0 ? 1.0 : 0.0);
+  bool alok = true;                                  // (PA: a non-finite exponent computes with 1 and stores NaN)
+  if (PA) {
+    al = lane < nm ? alv[rows.m(lane < nm ? lane : 0)] : 1.0;
+    alok = fabs(al) <= 1.7976931348623157e308;
+    al = alok ? al : 1.0;
+  }
+  __syncthreads();
+
+  // ---- phase 1
+  GmfOut o;
+  gmf_lane<KIND, ORDER, PA>(a, al, bt, o);
+  if (PA) {                                          // (J and H carry the factors sc1, sc2)
+    const double nanf = alok ? 1.0 : dnan();
+#pragma unroll
+    for (int k = 0; k < 9; k++) o.F[k] *= nanf;
+    o.sc1 *= nanf;
+    o.sc2 *= nanf;
+  }
+  if (ORDER >= 1) {
+    OT* my = mainr + lane * L::MS;
+#pragma unroll
+    for (int k = 0; k < 9; k++) { my[k] = (OT)o.r[1][k]; my[9 + k] = (OT)o.r[2][k]; }
+    if (ORDER >= 2) {
+#pragma unroll
+      for (int i = 0; i < 3; i++)
+#pragma unroll
+        for (int j = i; j < 3; j++)
+#pragma unroll
+          for (int k = j; k < 3; k++) my[18 + sym33s(i, j, k)] = (OT)o.D3[9 * i + 3 * j + k];
+      my[28] = (OT)o.sc2;
+    }
+  }
+#pragma unroll
+  for (int k = 0; k < 9; k++) xr[lane * 9 + k] = (OT)o.F[k];
+  __syncthreads();
+  copy_out<9, false, R_IN>(F, rows, 9 * nm, lane, [&](int j, int e) { return xr[9 * j + e]; });
+  if (ORDER >= 1) {
+    __syncthreads();
+#pragma unroll
+    for (int i = 0; i < 3; i++)
+#pragma unroll
+      for (int j = i; j < 3; j++) xr[L::XAUX + lane * 7 + sym3(i, j)] = (OT)o.D2[3 * i + j];
+    xr[L::XAUX + lane * 7 + 6] = (OT)o.sc1;
+    __syncthreads();
+
+    // ---- phase 2: lane (g, Lg) = matrix g of the pass, position Lg = 9 u + 3 v + w of a 27-lane group
+    constexpr int G = 27, MPP = WAVE / G;
+    const int g = lane / G, Lg = lane - g * G;
+    for (int p0 = 0; p0 < nm; p0 += MPP) {          // J: J_abcd for a = 0..2 at 27 a + (9b + 3c + d)
+      const int cnt = nm - p0 < MPP ? nm - p0 : MPP;
+      if (g < cnt) {
+        const int j = p0 + g;
+        const OT* s = mainr + j * L::MS;
+        const OT* aux = xr + L::XAUX + j * 7;
+        OT out[3];
+        gmf_j_lane(s, s + 9, aux, aux[6], Lg / 9, (Lg / 3) % 3, Lg % 3, out);
+#pragma unroll
+        for (int k = 0; k < 3; k++) xr[g * 81 + 27 * k + Lg] = out[k];
+      }
+      __syncthreads();
+      copy_out<81, false, R_J>(J, shift_rows(rows, p0), 81 * cnt, lane, [&](int j, int e) { return xr[81 * j + e]; });
+      __syncthreads();
+    }
+    if (ORDER >= 2) {
+      for (int p0 = 0; p0 < nm; p0 += MPP) {        // H: H_abcdef, (a,b,c) = 0..26 at 27 (9a + 3b + c) + (9d + 3e + f)
+        const int cnt = nm - p0 < MPP ? nm - p0 : MPP;
+        OT out[27];
+        if (g < cnt) {
+          const OT* s = mainr + (p0 + g) * L::MS;
+          gmf_h_lane(s, s + 9, s + 18, s[28], Lg / 9, (Lg / 3) % 3, Lg % 3, out);
+        }
+        const R sub = shift_rows(rows, p0);
+#pragma unroll
+        for (int half = 0; half < 2; half++) {      // rows k < 14 (378 elements), then k >= 14 (351)
+          constexpr int K0[2] = {0, 14}, SZ[2] = {378, 351};
+          if (g < cnt) {
+#pragma unroll
+            for (int k = 0; k < 14; k++)
+              if (K0[half] + k < 27) xr[g * SZ[half] + 27 * k + Lg] = out[K0[half] + k];
+          }
+          __syncthreads();
+          if (R::RUN) {
+            for (int t = lane; t < SZ[half] * cnt; t += WAVE) {
+              const int gg = t >= SZ[half] ? 1 : 0, e = t - gg * SZ[half];
+              H[(sub.hr(0) + gg) * 729 + 27 * K0[half] + e] = xr[t];
+            }
+          } else {
+            for (int t = lane; t < SZ[half] * cnt; t += WAVE) {
+              const int gg = t >= SZ[half] ? 1 : 0, e = t - gg * SZ[half];
+              H[sub.hr(gg) * 729 + 27 * K0[half] + e] = xr[t];
+            }
+          }
+          __syncthreads();
+        }
+      }
+    }
+  }
+  __syncthreads();                                   // (the LDS is free for the next tile)
+}
+
+}  // namespace mf

@@ -1,0 +1,144 @@
+// smf_tile.hpp -- one tile (up to 64 symmetric 3x3 matrices, one wavefront) of the symmetric kernels:
+//   phase A, one matrix per lane: coalesced input through LDS, eigen-decomposition, F, the 21 distinct J values,
+//            the 10 second divided differences; F / J staged in LDS and written as one contiguous region per wave
+//   phase B, six lanes per matrix (one per Voigt row of H), ten matrices per pass: H rows from V and D2 staged in
+//            LDS, gathered into the API layout through a lookup table and written contiguously (a pass covers
+//            10 x 5.8 KB of consecutive output)
+// Every store instruction of the wave writes 64 consecutive elements: the output (96 B .. 5.8 KB per matrix) bounds
+// the kernel by HBM write bandwidth, which needs whole cache lines per instruction.
+// OT = double: the FP64 form; OT = float: the mixed form (FP64 eigen-decomposition and divided differences, FP32
+// derivative assembly and FP32 outputs).  PA: powers with one exponent per matrix (alv[i], read by its lane; a
+// matrix whose exponent is not finite gets NaN results).  Used by the homogeneous kernels (smf_kernels.hip, RunRows)
+// and the heterogeneous batch kernel (batch.hip, ListRows).
+#pragma once
+#include "smf_device.hpp"
+#include "tile_rows.hpp"
+
+namespace mf {
+
+constexpr int SA = 27;             // phase-A tile per matrix: F (6) + Js (21)
+constexpr int SB = 19;             // V (9) + D2 (10)
+constexpr int HPP = 10;            // matrices per phase-B pass (6 lanes each)
+constexpr int SH = 127;            // staged H values per matrix (6 rows x 21, +1: odd stride)
+
+// LDS of one tile, in bytes: region 1 (input block, phase-A tile, H tiles) and the V / D2 block
+template <bool MAT, int ORDER, class OT>
+struct SmfLds {
+  static constexpr int NIN = MAT ? 9 : 6;
+  static constexpr int R1 = (WAVE * SA > HPP * SH ? WAVE * SA : HPP * SH);
+  static constexpr int R1B = (R1 * (int)sizeof(OT) > WAVE * NIN * 8 ? R1 * (int)sizeof(OT) : WAVE * NIN * 8);
+  static constexpr int TBB = ORDER >= 2 ? WAVE * SB * (int)sizeof(OT) : 0;
+  static constexpr int BYTES = ((R1B + 15) / 16) * 16 + TBB;
+};
+constexpr int smf_nj(bool mat) { return mat ? 81 : 36; }
+constexpr int smf_nh(bool mat) { return mat ? 729 : 216; }
+
+// lookup tables of the output layouts: distinct-value position | weight code << 8 (J: weight 2 if code 3; H: weight
+// 2^-code), filled by the wave
+template <bool MAT, int ORDER>
+MF_DEV void smf_luts(unsigned short* lutJ, unsigned short* lutH, int lane) {
+  if (ORDER >= 1)
+    for (int e = lane; e < smf_nj(MAT); e += WAVE) {
+      double w;
+      const int s = j_src(MAT, e, w);
+      lutJ[e] = (unsigned short)(s | (w == 2.0 ? 3 << 8 : 0));
+    }
+  if (ORDER >= 2)
+    for (int e = lane; e < smf_nh(MAT); e += WAVE) {
+      double w;
+      const int s = h_src(MAT, e, w);
+      lutH[e] = (unsigned short)(s | ((w == 1.0 ? 0 : (w == 0.5 ? 1 : 2)) << 8));
+    }
+}
+
+// one tile of nm <= 64 matrices; lds: SmfLds<MAT, ORDER, OT>::BYTES bytes, 16-byte aligned; the LUTs filled
+// (smf_luts) and visible to the wave; lane = threadIdx.x.  Ends with the tile's outputs written and a barrier (the
+// LDS may be reused).
+template <int KIND, bool MAT, int ORDER, class OT, bool PAIRS, bool PA, class R>
+MF_DEV void smf_tile(const R& rows, int nm, const double* __restrict__ x, OT* __restrict__ F, OT* __restrict__ J,
+                     OT* __restrict__ H, double al, const double* __restrict__ alv, const double* __restrict__ c2,
+                     unsigned char* lds, const unsigned short* lutJ, const unsigned short* lutH, int lane) {
+  using L = SmfLds<MAT, ORDER, OT>;
+  constexpr int NIN = L::NIN, NF = MAT ? 9 : 6, NJ = NF * NF, NH = NJ * NF;
+  double* xin = reinterpret_cast<double*>(lds);
+  OT* t1 = reinterpret_cast<OT*>(lds);
+  OT* tb = reinterpret_cast<OT*>(lds + ((L::R1B + 15) / 16) * 16);
+
+  // ---- input, coalesced through LDS (lanes past the end take the identity)
+  load_in<NIN>(xin, x, rows, nm, lane);
+  __syncthreads();
+  double v[6];
+  if (MAT) {
+    double a[9];
+#pragma unroll
+    for (int k = 0; k < 9; k++) a[k] = lane < nm ? xin[9 * lane + k] : (k % 4 == 0 ? 1.0 : 0.0);
+    matrix_to_voigt(a, v);
+  } else {
+#pragma unroll
+    for (int k = 0; k < 6; k++) v[k] = lane < nm ? xin[6 * lane + k] : (k < 3 ? 1.0 : 0.0);
+  }
+  bool alok = true;                            // (PA: a non-finite exponent computes with 1 and stores NaN)
+  if (PA) {
+    al = lane < nm ? alv[rows.m(lane < nm ? lane : 0)] : 1.0;
+    alok = fabs(al) <= 1.7976931348623157e308;
+    al = alok ? al : 1.0;
+  }
+  const double nanf = alok ? 1.0 : dnan();
+  __syncthreads();
+
+  // ---- phase A
+  {
+    SmfLane<KIND, OT> s;
+    double Fv[6];
+    s.eigen(v, al, Fv);
+#pragma unroll
+    for (int k = 0; k < 6; k++) t1[lane * SA + k] = (OT)(Fv[k] * nanf);
+    if (ORDER >= 1) {
+      OT Js[21];
+      s.jacobian(al, Js);
+#pragma unroll
+      for (int k = 0; k < 21; k++) t1[lane * SA + 6 + k] = Js[k] * (OT)nanf;
+    }
+    if (ORDER >= 2) {
+      s.template second_dd<PA>(al, c2);
+#pragma unroll
+      for (int k = 0; k < 9; k++) tb[lane * SB + k] = (OT)s.V[k / 3][k % 3];
+#pragma unroll
+      for (int k = 0; k < 10; k++) tb[lane * SB + 9 + k] = (OT)(s.D2[k] * nanf);
+    }
+  }
+  __syncthreads();
+  copy_out<NF, false, R_IN>(F, rows, NF * nm, lane, [&](int j, int e) { return t1[j * SA + f_src(MAT, e)]; });
+  if (ORDER >= 1)
+    copy_out<NJ, PAIRS, R_J>(J, rows, NJ * nm, lane, [&](int j, int e) {
+      const int u = lutJ[e];
+      const OT val = t1[j * SA + 6 + (u & 255)];
+      return (u >> 8) ? (OT)2 * val : val;
+    });
+  if (ORDER >= 2) {
+    // ---- phase B: lane = (matrix jj of the pass, Voigt row i)
+    const int jj = lane / 6, ir = lane - 6 * jj;
+    for (int p0 = 0; p0 < nm; p0 += HPP) {
+      const int cnt = nm - p0 < HPP ? nm - p0 : HPP;
+      __syncthreads();                         // (previous tile fully read)
+      if (jj < cnt) {
+        const OT* sb = tb + (p0 + jj) * SB;
+        const SmfHess<OT> hs(sb, sb + 9);
+        OT h[21];
+        hs.h_row(ir, h);
+#pragma unroll
+        for (int k = 0; k < 21; k++) t1[jj * SH + ir * 21 + k] = h[k];
+      }
+      __syncthreads();
+      const R sub = shift_rows(rows, p0);
+      copy_out<NH, PAIRS, R_H>(H, sub, NH * cnt, lane, [&](int j, int e) {
+        const int u = lutH[e];
+        const OT val = t1[j * SH + (u & 255)];
+        return (u >> 8) == 0 ? val : ((u >> 8) == 1 ? (OT)0.5 * val : (OT)0.25 * val);
+      });
+    }
+  }
+  __syncthreads();                             // (the LDS is free for the next tile)
+}
+
+}  // namespace mf

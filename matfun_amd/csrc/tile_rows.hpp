@@ -1,0 +1,107 @@
+// tile_rows.hpp -- where the up to 64 matrices of one tile (one wavefront) live in memory.
+//   RunRows   consecutive matrices base .. base + nm - 1 (the C ABI's homogeneous calls): the input / F, J and H rows
+//             of tile matrix j are all base + j, so every region a tile writes is one contiguous block
+//   ListRows  rows listed per matrix (heterogeneous batches): input / F row m[j], J row jr[j], H row hr[j], copied
+//             into LDS at the start of the tile
+// The tile bodies (smf_tile.hpp, gmf_tile.hpp) take either; with RunRows they compile to the contiguous addressing.
+#pragma once
+#include "mf_device.hpp"
+
+namespace mf {
+
+constexpr int WAVE = 64;
+
+struct RunRows {
+  static constexpr bool RUN = true;
+  int64_t base;
+  MF_DEV int64_t m(int j) const { return base + j; }
+  MF_DEV int64_t jr(int j) const { return base + j; }
+  MF_DEV int64_t hr(int j) const { return base + j; }
+};
+
+struct ListRows {
+  static constexpr bool RUN = false;
+  const int64_t* mr;              // LDS
+  const int64_t* jrr;
+  const int64_t* hrr;
+  MF_DEV int64_t m(int j) const { return mr[j]; }
+  MF_DEV int64_t jr(int j) const { return jrr[j]; }
+  MF_DEV int64_t hr(int j) const { return hrr[j]; }
+};
+
+// a copy of v / p that the compiler cannot see through (an empty asm statement): what a tile computes from it is not
+// hoisted out of a loop over tiles, so a persistent kernel keeps no per-tile state live from one tile to the next
+MF_DEV int opaque(int v) {
+  asm volatile("" : "+v"(v));
+  return v;
+}
+template <class T>
+MF_DEV const T* opaque(const T* p) {
+  asm volatile("" : "+s"(p));
+  return p;
+}
+
+// the rows of tile matrices p0, p0 + 1, ... (a pass over part of the tile)
+MF_DEV RunRows shift_rows(const RunRows& r, int p0) { return RunRows{r.base + p0}; }
+MF_DEV ListRows shift_rows(const ListRows& r, int p0) { return ListRows{r.mr + p0, r.jrr + p0, r.hrr + p0}; }
+
+// which row of a tile matrix an output region uses
+enum Region { R_IN = 0, R_J = 1, R_H = 2 };
+template <int REG, class R>
+MF_DEV int64_t row_of(const R& r, int j) {
+  return REG == R_IN ? r.m(j) : (REG == R_J ? r.jr(j) : r.hr(j));
+}
+
+// out[row(j) N + e] = get(j, e) for the count = N nm elements (t = j N + e) of the tile's matrices j = 0 .. nm-1 in
+// region REG, by the wave.  RunRows: one contiguous block (out + base N + t); with PAIRS two consecutive elements per
+// lane and one vector store (16 B FP64 / 8 B FP32; the block must be aligned to it), else one element per lane.
+// ListRows: one element per lane, per-matrix row from LDS.
+template <int N, bool PAIRS, int REG, class R, class OT, class Get>
+MF_DEV void copy_out(OT* __restrict__ out, const R& rows, int count, int lane, Get get) {
+  if (R::RUN && PAIRS) {
+    typedef OT v2 __attribute__((ext_vector_type(2)));
+    OT* blk = out + row_of<REG>(rows, 0) * N;
+    int t = 2 * lane, j = t / N, e = t - j * N;
+    for (; t + 1 < count; t += 2 * WAVE) {
+      const int e1 = e + 1 == N ? 0 : e + 1, j1 = e + 1 == N ? j + 1 : j;
+      const v2 val = {get(j, e), get(j1, e1)};
+      *reinterpret_cast<v2*>(blk + t) = val;
+      e += 2 * WAVE;
+      while (e >= N) { e -= N; j++; }
+    }
+    if (t < count) blk[t] = get(j, e);         // (odd count: the last element)
+  } else if (R::RUN) {
+    OT* blk = out + row_of<REG>(rows, 0) * N;
+    int j = lane / N, e = lane - j * N;
+    for (int t = lane; t < count; t += WAVE) {
+      blk[t] = get(j, e);
+      e += WAVE;
+      while (e >= N) { e -= N; j++; }
+    }
+  } else {
+    int j = lane / N, e = lane - j * N;
+    for (int t = lane; t < count; t += WAVE) {
+      out[row_of<REG>(rows, j) * N + e] = get(j, e);
+      e += WAVE;
+      while (e >= N) { e -= N; j++; }
+    }
+  }
+}
+
+// the tile's inputs (NIN doubles per matrix) into LDS, coalesced
+template <int NIN, class R>
+MF_DEV void load_in(double* __restrict__ xin, const double* __restrict__ x, const R& rows, int nm, int lane) {
+  if (R::RUN) {
+    const double* blk = x + rows.m(0) * NIN;
+    for (int t = lane; t < NIN * nm; t += WAVE) xin[t] = blk[t];
+  } else {
+    int j = lane / NIN, e = lane - j * NIN;
+    for (int t = lane; t < NIN * nm; t += WAVE) {
+      xin[t] = x[rows.m(j) * NIN + e];
+      e += WAVE;
+      while (e >= NIN) { e -= NIN; j++; }
+    }
+  }
+}
+
+}  // namespace mf

@@ -4,6 +4,7 @@
 #include <stdint.h>
 
 #include "gmf_device.hpp"
+#include "philox.hpp"
 #include "smf_device.hpp"
 
 namespace {
@@ -142,4 +143,16 @@ extern "C" int lanecheck_smf_matrix_pow(const double* alv, int64_t n, const doub
 extern "C" int lanecheck_gmf_pow(const double* alv, int64_t n, const double* A, double* F, double* J, double* H) {
   gmf_order<mf::POW, true>(H ? 2 : (J ? 1 : 0), 1.0, alv, n, A, F, J, H);
   return 0;
+}
+
+// Philox4x32-10 and the perturbation factors of the Newton driver (philox.hpp), on the CPU
+extern "C" void lanecheck_philox(const uint32_t* ctr, const uint32_t* key, uint32_t* out) {
+  mf::U4 c;
+  for (int k = 0; k < 4; k++) c.x[k] = ctr[k];
+  const mf::U4 r = mf::philox4x32_10(c, key[0], key[1]);
+  for (int k = 0; k < 4; k++) out[k] = r.x[k];
+}
+extern "C" void lanecheck_perturb(uint64_t seed, uint32_t iteration, int64_t n, const int32_t* sym, double sigma,
+                                  double* f9) {
+  for (int64_t m = 0; m < n; m++) mf::perturb_factors(seed, iteration, (uint64_t)m, sym[m] != 0, sigma, f9 + 9 * m);
 }

@@ -1,0 +1,217 @@
+"""Heterogeneous batches (mf_hip_plan_* / mf_hip_batch_eval*, an extension without counterpart in the reference API)
+on the GPU.  A batch evaluates each matrix with the tile code of the homogeneous entry points, so the contract is
+bitwise equality with them: every group's F / J / H equal smf_hip_eval_matrix / gmf_hip_eval on the same matrices
+(pow: smf_hip_eval_matrix_pow / gmf_hip_eval_pow with the same exponents), in both precisions, for every launch
+strategy.  The homogeneous entry points are pinned to the oracle by the other GPU tests.  Newton driver: the Philox
+perturbation against its CPU build (the generator is pinned to the published known answers on the CPU), and a
+graph-captured sequence against the same steps launched eagerly, over several replays with eager work between."""
+import ctypes
+import os
+
+import numpy as np
+import pytest
+
+from matfun_amd import workloads as W
+
+pytestmark = pytest.mark.gpu
+KIND = {"exp": 0, "log": 1, "pow": 2, "sqrt": 3}
+
+
+def groups_all_classes(seed):
+    """every (family, function, order), counts 0 .. 70 (tile boundaries 63 / 64 / 65 among them), shuffled"""
+    rng = np.random.default_rng(seed)
+    gs = []
+    for fam in ("smf", "gmf"):
+        for kind in ("exp", "log", "sqrt", "pow"):
+            for order in (0, 1, 2):
+                for count in rng.choice([0, 1, 5, 27, 63, 64, 65, 70], 3, replace=False):
+                    gs.append((fam, kind, order, int(count), float(rng.uniform(-1.0, 1.0))))
+    rng.shuffle(gs)
+    return gs
+
+
+def inputs(groups, seed):
+    """A (N, 9): SPD matrices for smf groups (with repeated eigenvalues), general ones for gmf groups"""
+    n = sum(g[3] for g in groups)
+    S = W.spd(max(n, 1), seed=seed)
+    G = W.general(max(n, 1), seed=seed + 1)
+    A = np.empty((n, 9))
+    lo = 0
+    for g in groups:
+        A[lo:lo + g[3]] = (S if g[0] == "smf" else G)[lo:lo + g[3]]
+        lo += g[3]
+    return A
+
+
+def reference(groups, A, alpha, mixed):
+    """the homogeneous calls group by group; returns F, J, H stacked in the batch layout"""
+    from tests.gpu_util import call, call_pow
+    Fs, Js, Hs = [], [], []
+    lo = 0
+    for fam, kind, order, count, al in groups:
+        if count == 0:
+            continue
+        X = A[lo:lo + count]
+        family = "matrix" if fam == "smf" else "general"
+        if kind == "pow":
+            a = alpha[lo:lo + count] if alpha is not None else np.full(count, al)
+            rc, F, J, H = call_pow(family, a, X, order, mixed=mixed)
+        else:
+            rc, F, J, H = call(family, kind, 0.0, X, order, mixed=mixed)
+        assert rc == 0
+        Fs.append(F)
+        if order >= 1:
+            Js.append(J)
+        if order >= 2:
+            Hs.append(H)
+        lo += count
+    cat = lambda xs, w: np.concatenate(xs) if xs else np.zeros((0, w))
+    return cat(Fs, 9), cat(Js, 81), cat(Hs, 729)
+
+
+def run_batch(b, A, alpha, mixed):
+    import torch
+    dA = torch.from_numpy(A).cuda()
+    dal = None if alpha is None else torch.from_numpy(alpha).cuda()
+    F, J, H = b.eval(dA, dal, precision="mixed" if mixed else "fp64")
+    torch.cuda.synchronize()
+    g = lambda t: t.reshape(t.shape[0], -1).cpu().double().numpy()
+    return g(F), g(J), g(H)
+
+
+def same(x, y):
+    return x.shape == y.shape and np.array_equal(x, y, equal_nan=True)
+
+
+@pytest.mark.parametrize("mixed", [False, True])
+@pytest.mark.parametrize("per_matrix_alpha", [False, True])
+def test_batch_equals_homogeneous_calls(mixed, per_matrix_alpha):
+    import matfun_amd
+    gs = groups_all_classes(601)
+    A = inputs(gs, 602)
+    alpha = np.random.default_rng(603).uniform(-1.0, 1.0, len(A)) if per_matrix_alpha else None
+    b = matfun_amd.Batch(gs)
+    assert (b.n, b.nclasses) == (len(A), 24)
+    print(f"batch n={b.n} nj={b.nj} nh={b.nh} tiles={b.ntiles} mixed={mixed} alpha={per_matrix_alpha}", flush=True)
+    got = run_batch(b, A, alpha, mixed)
+    ref = reference(gs, A, alpha, mixed)
+    for name, x, y in zip("FJH", got, ref):
+        assert same(x, y), (name, x.shape, y.shape, np.nanmax(np.abs(x - y)) if x.shape == y.shape else None)
+    b.close()
+
+
+def test_every_strategy_gives_the_same_results():
+    import matfun_amd
+    gs = groups_all_classes(604)
+    A = inputs(gs, 605)
+    b = matfun_amd.Batch(gs)
+    base = None
+    for mode, per_cu, order in [(0, 0, 0), (0, 0, 1), (0, 2, 1), (0, 6, 0), (1, 0, 0), (1, 2, 0), (1, 8, 0)]:
+        print(f"strategy mode={mode} per_cu={per_cu} order={order}", flush=True)
+        b.set_strategy(mode, per_cu, order)
+        got = run_batch(b, A, None, False)
+        if base is None:
+            base = got
+        for x, y in zip(got, base):
+            assert same(x, y), (mode, per_cu, order)
+    b.close()
+
+
+def test_tuner_keeps_results_and_picks_a_strategy():
+    import torch
+    import matfun_amd
+    gs = groups_all_classes(606) * 4
+    A = inputs(gs, 607)
+    b = matfun_amd.Batch(gs)
+    before = run_batch(b, A, None, False)
+    dA = torch.from_numpy(A).cuda()
+    res = b.tune(dA)
+    print(f"tuned: {res}", flush=True)
+    assert res["mode"] in (0, 1) and res["us"] > 0
+    after = run_batch(b, A, None, False)
+    for x, y in zip(before, after):
+        assert same(x, y)
+    b.close()
+
+
+def test_perturbation_matches_cpu_build_and_keeps_symmetry():
+    import torch
+    import matfun_amd
+    lane = ctypes.CDLL(os.path.join(os.path.dirname(__file__), "lanecheck", "liblanecheck.so"))
+    lane.lanecheck_perturb.argtypes = [ctypes.c_uint64, ctypes.c_uint32, ctypes.c_int64, ctypes.c_void_p,
+                                       ctypes.c_double, ctypes.c_void_p]
+    gs = groups_all_classes(608)
+    A = inputs(gs, 609)
+    b = matfun_amd.Batch(gs)
+    dA = torch.from_numpy(A).cuda()
+    b.perturb(dA, 1e-3, 987654321, 7)
+    got = dA.cpu().numpy()
+    sym = np.concatenate([np.full(g[3], 1 if g[0] == "smf" else 0, np.int32) for g in gs])
+    f = np.empty((len(A), 9))
+    lane.lanecheck_perturb(987654321, 7, len(A), sym.ctypes.data, 1e-3, f.ctypes.data)
+    want = A * f
+    # (Box-Muller's log / cos / sin: device and host libraries may differ in the last bit)
+    assert np.all(np.abs(got - want) <= 4e-16 * np.abs(want))
+    S = got[sym == 1].reshape(-1, 3, 3)
+    assert np.array_equal(S, S.transpose(0, 2, 1))
+    b.close()
+
+
+@pytest.mark.parametrize("mixed", [False, True])
+def test_graph_sequence_equals_eager_steps(mixed):
+    import torch
+    import matfun_amd
+    gs = groups_all_classes(610)
+    A0 = inputs(gs, 611)
+    b = matfun_amd.Batch(gs)
+    prec = "mixed" if mixed else "fp64"
+    iters, sigma, seed = 20, 1e-3, 42
+    dA0 = torch.from_numpy(A0).cuda()
+    # eager: A <- A0, then iters x (perturb, eval)
+    dA = dA0.clone()
+    out_e = b.outputs(prec)
+    for it in range(1, iters + 1):
+        b.perturb(dA, sigma, seed, it)
+        b.eval(dA, out=out_e, precision=prec)
+    torch.cuda.synchronize()
+    eager = [t.cpu().numpy() for t in out_e]
+    eagerA = dA.cpu().numpy()
+    # graph: captured once, replayed three times with eager work and synchronisation between the replays
+    dA2 = torch.empty_like(dA0)
+    out_g = b.outputs(prec)
+    seq = b.sequence(dA0, dA2, out_g, iters, sigma, seed, precision=prec)
+    for rep in range(3):
+        print(f"graph replay {rep} mixed={mixed}", flush=True)
+        for t in out_g:
+            t.fill_(7.0)
+        torch.cuda.synchronize()
+        seq.launch()
+        torch.cuda.synchronize()
+        assert np.array_equal(dA2.cpu().numpy(), eagerA)
+        for x, y in zip(out_g, eager):
+            assert np.array_equal(x.cpu().numpy(), y, equal_nan=True), rep
+        b.eval(dA0, out=out_e, precision=prec)           # (eager work between replays)
+        torch.cuda.synchronize()
+    seq.close()
+    b.close()
+
+
+def test_invalid_plans_and_calls():
+    import torch
+    import matfun_amd
+    for bad in ([("smf", "log", 3, 4)], [("xmf", "log", 1, 4)], [("smf", "tan", 1, 4)], [("smf", "log", 1, -1)],
+                [("gmf", "pow", 1, 4, float("nan"))]):
+        with pytest.raises((ValueError, KeyError, TypeError)):
+            matfun_amd.Batch(bad)
+    b = matfun_amd.Batch([("smf", "log", 2, 10)])
+    A = torch.from_numpy(W.spd(10, seed=612)).cuda()
+    F, J, H = b.outputs()
+    lib = matfun_amd.library()
+    assert lib.mf_hip_batch_eval(b._h, A.data_ptr(), None, F.data_ptr(), J.data_ptr(), None, None) == -1
+    assert lib.mf_hip_batch_eval(b._h, A.data_ptr(), None, F.data_ptr(), None, H.data_ptr(), None) == -1
+    assert lib.mf_hip_batch_eval(None, A.data_ptr(), None, F.data_ptr(), J.data_ptr(), H.data_ptr(), None) == -1
+    empty = matfun_amd.Batch([])
+    assert empty.n == 0
+    empty.eval(None, out=(torch.empty(0), torch.empty(0), torch.empty(0)))
+    b.close()
+    empty.close()

@@ -164,3 +164,36 @@ def test_golden_vectors(lane):
         F, J, H = run(lane.lanecheck_gmf, g["kind"][i], g["alpha"][i], g["A"][i][None], 9, 2)
         for x, ref in ((F[0], g["F"][i]), (J[0], g["J"][i]), (H[0], g["H"][i])):
             assert np.max(np.abs(x - ref)) / np.max(np.abs(ref)) <= g["tol"][i], (i, str(g["label"][i]))
+
+
+def test_philox4x32_10_known_answers(lane):
+    """the generator of the Newton driver's perturbations against the known-answer vectors published with Philox
+    (Salmon et al., SC'11; Random123 kat_vectors: philox4x32 10 rounds)"""
+    lane.lanecheck_philox.argtypes = [ctypes.c_void_p] * 3
+    kat = [([0, 0, 0, 0], [0, 0], [0x6627e8d5, 0xe169c58d, 0xbc57ac4c, 0x9b00dbd8]),
+           ([0xffffffff] * 4, [0xffffffff] * 2, [0x408f276d, 0x41c83b0e, 0xa20bc7c6, 0x6d5451fd]),
+           ([0x243f6a88, 0x85a308d3, 0x13198a2e, 0x03707344], [0xa4093822, 0x299f31d0],
+            [0xd16cfe09, 0x94fdcceb, 0x5001e420, 0x24126ea1])]
+    for ctr, key, want in kat:
+        c, k, o = (np.array(ctr, np.uint32), np.array(key, np.uint32), np.zeros(4, np.uint32))
+        lane.lanecheck_philox(c.ctypes.data, k.ctypes.data, o.ctypes.data)
+        assert [int(x) for x in o] == want
+
+
+def test_perturbation_factors_are_normal_and_symmetric(lane):
+    lane.lanecheck_perturb.argtypes = [ctypes.c_uint64, ctypes.c_uint32, ctypes.c_int64, ctypes.c_void_p,
+                                       ctypes.c_double, ctypes.c_void_p]
+    n = 200000
+    sym = np.zeros(n, np.int32)
+    sym[::2] = 1
+    f = np.empty((n, 9))
+    lane.lanecheck_perturb(12345, 3, n, sym.ctypes.data, 1.0, f.ctypes.data)
+    z = f - 1.0
+    zs = z[1::2].ravel()                                 # general matrices: 9 independent normals each
+    assert abs(zs.mean()) < 0.01 and abs(zs.var() - 1.0) < 0.01
+    assert abs(np.mean(zs ** 4) - 3.0) < 0.05           # (kurtosis of a normal)
+    Z = z[::2].reshape(-1, 3, 3)                        # symmetric ones: (i, j) and (j, i) share the normal
+    assert np.array_equal(Z, Z.transpose(0, 2, 1))
+    f2 = np.empty((n, 9))
+    lane.lanecheck_perturb(12345, 4, n, sym.ctypes.data, 1.0, f2.ctypes.data)
+    assert abs(np.corrcoef(f[1::2, 0], f2[1::2, 0])[0, 1]) < 0.01    # iterations draw independent streams
