@@ -11,9 +11,12 @@ exp of eigenvalues up to 1e2 is beyond the FP32 range) with its error against th
 Key "mixed_vs_fp64": the mixed form against the FP64 form on smf log and smf pow with one exponent per matrix drawn
 uniform in [-1, 1] (order 2, n = 2^24, the headline distribution, seed 1) plus gmf log order 1 (n = 2^22, the general
 mix, seed 1): time per step of each form, the speed-up, and the mixed form's error against the FP64 form per output.
+Key "hetero_batch": a heterogeneous batch of 2^20 finite elements (U{1,4,8,10,27} points; smf log o2 / smf pow o1
+with per-matrix exponents / gmf exp o1 / gmf log o2) evaluated by one kernel, 20 Newton iterations per hipGraph
+replay, launch strategy tuned, FP64 and mixed (hetero_batch's docstring).
 
     python bench.py [--steps K] [--warmup W] [--n N] [--gmf-n N] [--gpus 1] [--no-cpu] [--only-headline]
-                    [--mixed-n N] [--mixed-gmf-n N]
+                    [--mixed-n N] [--mixed-gmf-n N] [--batch-elements E] [--batch-iters I]
 
 `value` = matrices/s of the headline step (4 n matrices per step).  `roofline`: the dominant kernel (the smf order-2
 matrix-layout FP64 kernel): algorithmic bytes per launch (A in + F, J, H out: 6624 B per matrix) over its average
@@ -176,6 +179,72 @@ def mixed_vs_fp64(torch, lib, W, dev, stream, args, sample=1 << 16):
     return res
 
 
+def hetero_batch(torch, W, dev, stream, args):
+    """Heterogeneous persistent batch: E finite elements with U{1, 4, 8, 10, 27} integration points each; per element
+    40% smf log order 2, 20% smf pow order 1 (exponent per matrix ~ U[-1, 1]), 20% gmf exp order 1, 20% gmf log
+    order 2; inputs as the headline (smf) and general (gmf) workloads (seed 2 / 3); one step = one replay of a hipGraph
+    of `iters` Newton iterations (relative N(0, 1e-3) perturbation of A, then the batch evaluation), launch strategy
+    tuned first; FP64 and mixed outputs"""
+    import matfun_amd
+    E, iters = args.batch_elements, args.batch_iters
+    rng = np.random.default_rng(2)
+    pts = rng.choice(np.array([1, 4, 8, 10, 27]), E)
+    cls = rng.choice(4, E, p=[0.4, 0.2, 0.2, 0.2])      # smf log o2, smf pow o1, gmf exp o1, gmf log o2
+    fam = np.where(cls < 2, 0, 1)
+    kind = np.array([1, 2, 0, 1])[cls]
+    order = np.array([2, 1, 1, 2])[cls]
+    groups = matfun_amd.group_array(fam, kind, order, pts)
+    N = int(pts.sum())
+    is_smf = np.repeat(fam == 0, pts)
+    log(f"bench: batch: {E} elements, {N} matrices; generating inputs (seed 2 / 3)")
+    A = np.empty((N, 9))
+    A[is_smf] = W.spd(int(is_smf.sum()), seed=2)
+    A[~is_smf] = W.general(int((~is_smf).sum()), seed=3)
+    alpha = np.random.default_rng(4).uniform(-1.0, 1.0, N)
+    b = matfun_amd.Batch(groups)
+    dA0 = torch.from_numpy(A).to(dev)
+    dA = torch.empty_like(dA0)
+    dal = torch.from_numpy(alpha).to(dev)
+    nJ, nH = b.nj, b.nh
+    res = {"workload": f"{E} elements x U{{1,4,8,10,27}} points = {N} matrices; 40% smf log o2, 20% smf pow o1 "
+                       f"(alpha per matrix ~ U[-1,1]), 20% gmf exp o1, 20% gmf log o2; {iters} Newton iterations "
+                       f"(A <- A o (1 + N(0, 1e-3)), Philox4x32-10) per hipGraph replay",
+           "elements": E, "matrices": N, "j_rows": nJ, "h_rows": nH, "tiles": b.ntiles, "iterations": iters}
+    for prec, eb in (("fp64", 8), ("mixed", 4)):
+        out = b.outputs(prec)
+        dA.copy_(dA0)
+        tuned = b.tune(dA, dal, out=out, precision=prec)
+        seq = b.sequence(dA0, dA, out, iters, 1e-3, 12345, alpha=dal, precision=prec)
+        for _ in range(args.warmup):
+            seq.launch()
+        torch.cuda.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record(stream)
+        for _ in range(args.steps):
+            seq.launch()
+        e1.record(stream)
+        torch.cuda.synchronize()
+        ms = e0.elapsed_time(e1) / args.steps
+        # algorithmic bytes per iteration: A in (72 B) + F out + J / H rows out + the perturbation's read and write of
+        # A (144 B) + 8 B of exponent per pow matrix
+        npow = int(pts[cls == 1].sum())
+        byts = N * (72 + 144 + 9 * eb) + nJ * 81 * eb + nH * 729 * eb + 8 * npow
+        t_it = ms / iters
+        res[prec] = {"ms_per_graph": ms, "ms_per_iteration": t_it, "matrices_per_s": N / (t_it * 1e-3),
+                     "bytes_per_matrix": byts / N, "achieved_GBs": byts / (t_it * 1e-3) / 1e9,
+                     "frac_hbm": byts / (t_it * 1e-3) / 1e9 / HBM_PEAK_GBS, "tuned_strategy": tuned}
+        log(f"bench: batch {prec}: {t_it:.3f} ms per iteration, {N / (t_it * 1e-3) / 1e6:.1f} M matrices/s, "
+            f"{res[prec]['frac_hbm']:.1%} of HBM peak, strategy {tuned}")
+        seq.close()
+        del out
+        torch.cuda.empty_cache()
+    res["speedup_mixed_vs_fp64"] = res["fp64"]["ms_per_graph"] / res["mixed"]["ms_per_graph"]
+    b.close()
+    del dA0, dA, dal
+    torch.cuda.empty_cache()
+    return res
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -188,6 +257,8 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--mixed-n", type=int, default=1 << 24)
     ap.add_argument("--mixed-gmf-n", type=int, default=1 << 22)
+    ap.add_argument("--batch-elements", type=int, default=1 << 20)
+    ap.add_argument("--batch-iters", type=int, default=20)
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU only; --gpus must be 1 (the batch shards trivially "
@@ -292,6 +363,7 @@ def main():
         del Fb, Jb, Hb
         torch.cuda.empty_cache()
         legs_c = mixed_vs_fp64(torch, lib, W, dev, stream, args)
+        legs_b = hetero_batch(torch, W, dev, stream, args)
     # device copy bandwidth (read + write of 2 GiB), for context next to the 8 TB/s spec
     x = torch.empty(1 << 28, dtype=torch.float64, device=dev)
     y = torch.empty_like(x)
@@ -319,6 +391,7 @@ def main():
            "legs": legs}
     if not args.only_headline:
         out["mixed_vs_fp64"] = legs_c
+        out["hetero_batch"] = legs_b
     if not args.no_cpu:
         log("bench: CPU baseline")
         out["cpu_baseline"] = cpu_baseline(A, seconds=args.cpu_seconds)

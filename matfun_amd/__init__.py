@@ -190,21 +190,30 @@ class Batch:
     J holds a row (3, 3, 3, 3) per matrix of order >= 1 and H a row (3,)*6 per matrix of order 2, in group order.
     Device tensors in, device tensors out (torch's current stream)."""
 
+    GROUP_DTYPE = [("family", "<i4"), ("kind", "<i4"), ("order", "<i4"), ("count", "<i4"), ("alpha", "<f8")]
+
     def __init__(self, groups):
         lib = _batch_lib()
-        arr = (Group * max(1, len(groups)))()
-        for i, g in enumerate(groups):
-            fam, kind, order, count = g[:4]
-            alpha = g[4] if len(g) > 4 else 0.0
-            arr[i] = Group(FAMILIES.get(fam, fam) if isinstance(fam, str) else int(fam),
-                           KINDS[kind] if isinstance(kind, str) else int(kind), int(order), int(count), float(alpha))
+        if hasattr(groups, "dtype"):                   # a structured array of GROUP_DTYPE (large batches)
+            arr = groups
+            ng = len(groups)
+            ptr = ctypes.cast(arr.ctypes.data, ctypes.POINTER(Group)) if ng else (Group * 1)()
+        else:
+            ng = len(groups)
+            arr = ptr = (Group * max(1, ng))()
+            for i, g in enumerate(groups):
+                fam, kind, order, count = g[:4]
+                alpha = g[4] if len(g) > 4 else 0.0
+                arr[i] = Group(FAMILIES.get(fam, fam) if isinstance(fam, str) else int(fam),
+                               KINDS[kind] if isinstance(kind, str) else int(kind), int(order), int(count),
+                               float(alpha))
         h = ctypes.c_void_p()
-        r = lib.mf_hip_plan_create(arr, len(groups), ctypes.byref(h))
+        r = lib.mf_hip_plan_create(ptr, ng, ctypes.byref(h))
         if r != 0:
             raise ValueError(f"mf_hip_plan_create returned {r}")
         self._h, self._lib = h, lib
         self.n, self.nj, self.nh, self.ntiles, self.nclasses = (int(lib.mf_hip_plan_size(h, w)) for w in range(5))
-        self.groups = list(groups)
+        self.groups = groups
 
     def close(self):
         if self._h:
@@ -262,6 +271,16 @@ class Batch:
     def sequence(self, A0, A, out, iterations, sigma, seed, alpha=None, precision="fp64"):
         """a graph-captured Newton sequence (mf_hip_sequence_*): .launch() replays it on the current stream"""
         return Sequence(self, A0, A, out, iterations, sigma, seed, alpha, precision)
+
+
+def group_array(family, kind, order, count, alpha=None):
+    """groups as a structured array (Batch.GROUP_DTYPE) from equal-length integer arrays (and exponents)"""
+    import numpy as np
+    g = np.zeros(len(count), dtype=Batch.GROUP_DTYPE)
+    g["family"], g["kind"], g["order"], g["count"] = family, kind, order, count
+    if alpha is not None:
+        g["alpha"] = alpha
+    return g
 
 
 class Sequence:
