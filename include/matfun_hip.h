@@ -107,7 +107,10 @@ int mf_hip_batch_eval_mixed(const mf_hip_plan *plan, const double *A, const doub
 /* Launch strategy (every strategy gives bitwise the same results):
  *   mode 0 fused: one kernel over the tiles of all classes, tiles in class order (order 0) or interleaved in
  *          proportion to the classes' sizes (order 1)
- *   mode 1 split: one kernel per class (each compiled for its class alone)
+ *   mode 1 split: one kernel per class (each compiled for its class alone), one after another on `stream`
+ *   mode 2 split, concurrent: the class kernels on `stream` and up to 3 streams of the plan (forked from and joined
+ *          back to `stream` with events, so the call stays asynchronous and capturable; calls that share a plan
+ *          must not overlap)
  *   per_cu > 0: persistent, per_cu workgroups per compute unit loop over the tiles; 0: one workgroup per tile
  * mf_hip_plan_tune times the candidates on the caller's buffers (the batch is evaluated several times) and keeps the
  * fastest; `result` (optional) receives {mode, per_cu, order, microseconds of the best}.  Synchronous. */

@@ -49,6 +49,12 @@ struct mf_hip_plan {
   int64_t *d_rm = nullptr, *d_rj = nullptr, *d_rh = nullptr;
   double* d_alpha = nullptr;              // the groups' exponents per matrix (pow groups; 1 elsewhere)
   int mode = 0, per_cu = 6, order = 1;    // launch strategy
+  // mode 2: the classes on the caller's stream and up to NSIDE side streams (default priority), forked and joined
+  // with events; the classes' assignment balanced by estimated cost
+  static constexpr int NSIDE = 3;
+  hipStream_t side[NSIDE] = {};
+  hipEvent_t fork = nullptr, join[NSIDE] = {};
+  int lane_of_cls[NCLS] = {};             // 0: the caller's stream, k: side[k - 1]
 };
 
 struct mf_hip_sequence {
@@ -71,7 +77,18 @@ void free_plan(mf_hip_plan* p) {
   void* ptrs[] = {p->d_sorted, p->d_mixed, p->d_rm, p->d_rj, p->d_rh, p->d_alpha};
   for (void* q : ptrs)
     if (q) (void)hipFree(q);
+  for (int k = 0; k < mf_hip_plan::NSIDE; k++) {
+    if (p->side[k]) (void)hipStreamDestroy(p->side[k]);
+    if (p->join[k]) (void)hipEventDestroy(p->join[k]);
+  }
+  if (p->fork) (void)hipEventDestroy(p->fork);
   delete p;
+}
+
+// estimated time of one tile of a class, relative (order-2 smf = 1; from the homogeneous kernels' rates)
+double tile_cost(int c) {
+  const int o = cls_order(c);
+  return cls_fam(c) == 0 ? (o == 2 ? 1.0 : (o == 1 ? 0.15 : 0.05)) : (o == 2 ? 1.05 : (o == 1 ? 0.45 : 0.3));
 }
 
 BatchArgs args_of(const mf_hip_plan* p, const BatchTile* tiles, int64_t t0, int64_t t1, const double* A,
@@ -105,16 +122,30 @@ int batch_eval(const mf_hip_plan* p, const double* A, const double* alpha, OT* F
   if (mode == 0) {
     const BatchArgs a = args_of(p, order == 1 ? p->d_mixed : p->d_sorted, 0, p->ntiles, A, alpha);
     batch_launch_fused(st, grid_for(p, p->ntiles, per_cu), per_cu > 0, a, F, J, H);
-  } else {
-    for (int c = 0; c < NCLS; c++) {
-      const int64_t n = p->cls_t1[c] - p->cls_t0[c];
-      if (n == 0) continue;
-      const BatchArgs a = args_of(p, p->d_sorted, p->cls_t0[c], p->cls_t1[c], A, alpha);
-      if (cls_fam(c) == 0) batch_launch_smf(c, st, grid_for(p, n, per_cu), per_cu > 0, a, F, J, H);
-      else batch_launch_gmf(c, st, grid_for(p, n, per_cu), per_cu > 0, a, F, J, H);
-    }
+    return hipGetLastError() == hipSuccess ? 0 : -2;
   }
-  return hipGetLastError() == hipSuccess ? 0 : -2;
+  bool used[mf_hip_plan::NSIDE] = {};
+  if (mode == 2 && hipEventRecord(p->fork, st) != hipSuccess) return -2;
+  for (int c = 0; c < NCLS; c++) {
+    const int64_t n = p->cls_t1[c] - p->cls_t0[c];
+    if (n == 0) continue;
+    hipStream_t s = st;
+    if (mode == 2 && p->lane_of_cls[c] > 0) {
+      const int k = p->lane_of_cls[c] - 1;
+      if (!used[k] && hipStreamWaitEvent(p->side[k], p->fork, 0) != hipSuccess) return -2;
+      used[k] = true;
+      s = p->side[k];
+    }
+    const BatchArgs a = args_of(p, p->d_sorted, p->cls_t0[c], p->cls_t1[c], A, alpha);
+    if (cls_fam(c) == 0) batch_launch_smf(c, s, grid_for(p, n, per_cu), per_cu > 0, a, F, J, H);
+    else batch_launch_gmf(c, s, grid_for(p, n, per_cu), per_cu > 0, a, F, J, H);
+  }
+  const bool launched_ok = hipGetLastError() == hipSuccess;
+  for (int k = 0; k < mf_hip_plan::NSIDE; k++)     // (joined even after a launch error: no stream left forked)
+    if (used[k] && (hipEventRecord(p->join[k], p->side[k]) != hipSuccess ||
+                    hipStreamWaitEvent(st, p->join[k], 0) != hipSuccess))
+      return -2;
+  return launched_ok ? 0 : -2;
 }
 
 int slot_of(int family, int kind) {
@@ -189,6 +220,23 @@ extern "C" int mf_hip_plan_create(const mf_hip_group* groups, int64_t ngroups, m
   std::vector<BatchTile> mixed;
   mixed.reserve(p->tiles.size());
   for (auto& kv : key) mixed.push_back(p->tiles[(size_t)kv.second]);
+  // ---- mode 2: classes to streams, the largest estimated cost first onto the least loaded of 1 + NSIDE streams
+  {
+    std::vector<std::pair<double, int>> cost;
+    for (int c = 0; c < NCLS; c++)
+      if (p->cls_t1[c] > p->cls_t0[c]) cost.emplace_back(tile_cost(c) * (double)(p->cls_t1[c] - p->cls_t0[c]), c);
+    std::sort(cost.begin(), cost.end(), [](const std::pair<double, int>& x, const std::pair<double, int>& y) {
+      return x.first > y.first;
+    });
+    double load[1 + mf_hip_plan::NSIDE] = {};
+    for (auto& cc : cost) {
+      int best = 0;
+      for (int k = 1; k <= mf_hip_plan::NSIDE; k++)
+        if (load[k] < load[best]) best = k;
+      load[best] += cc.first;
+      p->lane_of_cls[cc.second] = best;
+    }
+  }
   // ---- series tables
   BatchTables tab;
   for (int s = 0; s < 3; s++) {
@@ -206,6 +254,16 @@ extern "C" int mf_hip_plan_create(const mf_hip_group* groups, int64_t ngroups, m
     free_plan(p);
     return -2;
   }
+  if (hipEventCreateWithFlags(&p->fork, hipEventDisableTiming) != hipSuccess) {
+    free_plan(p);
+    return -2;
+  }
+  for (int k = 0; k < mf_hip_plan::NSIDE; k++)
+    if (hipStreamCreateWithFlags(&p->side[k], hipStreamNonBlocking) != hipSuccess ||
+        hipEventCreateWithFlags(&p->join[k], hipEventDisableTiming) != hipSuccess) {
+      free_plan(p);
+      return -2;
+    }
   if (upload(&p->d_sorted, p->tiles.data(), p->tiles.size()) || upload(&p->d_mixed, mixed.data(), mixed.size()) ||
       upload(&p->d_rm, rm.data(), rm.size()) || upload(&p->d_rj, rj.data(), rj.size()) ||
       upload(&p->d_rh, rh.data(), rh.size()) || upload(&p->d_alpha, galpha.data(), galpha.size()) ||
@@ -245,7 +303,7 @@ extern "C" int mf_hip_batch_eval_mixed(const mf_hip_plan* p, const double* A, co
 }
 
 extern "C" int mf_hip_plan_set_strategy(mf_hip_plan* p, int mode, int per_cu, int order) {
-  if (!p || mode < 0 || mode > 1 || per_cu < 0 || per_cu > 64 || order < 0 || order > 1) return -1;
+  if (!p || mode < 0 || mode > 2 || per_cu < 0 || per_cu > 64 || order < 0 || order > 1) return -1;
   p->mode = mode;
   p->per_cu = per_cu;
   p->order = order;
@@ -262,6 +320,7 @@ extern "C" int mf_hip_plan_tune(mf_hip_plan* p, const double* A, const double* a
   for (int k : {0, 2, 4, 8})
     for (int o : {0, 1}) cands.push_back({0, k, o});
   for (int k : {0, 2, 4, 8}) cands.push_back({1, k, 0});
+  for (int k : {0, 2}) cands.push_back({2, k, 0});
   hipStream_t st = (hipStream_t)stream;
   hipEvent_t e0, e1;
   if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess) return -2;

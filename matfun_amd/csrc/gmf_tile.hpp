@@ -132,10 +132,9 @@ MF_DEV void gmf_tile(const R& rows, int nm, const double* __restrict__ A, OT* __
               H[(sub.hr(0) + gg) * 729 + 27 * K0[half] + e] = xr[t];
             }
           } else {
-            for (int t = lane; t < SZ[half] * cnt; t += WAVE) {
-              const int gg = t >= SZ[half] ? 1 : 0, e = t - gg * SZ[half];
-              H[sub.hr(gg) * 729 + 27 * K0[half] + e] = xr[t];
-            }
+            OT* h0 = H + sub.hr(0) * 729 + 27 * K0[half];
+            OT* h1 = cnt > 1 ? H + sub.hr(1) * 729 + 27 * K0[half] : h0;
+            for (int t = lane; t < SZ[half] * cnt; t += WAVE) *(t >= SZ[half] ? h1 + (t - SZ[half]) : h0 + t) = xr[t];
           }
           __syncthreads();
         }

@@ -78,6 +78,21 @@ MF_DEV void copy_out(OT* __restrict__ out, const R& rows, int count, int lane, G
       e += WAVE;
       while (e >= N) { e -= N; j++; }
     }
+  } else if (N >= WAVE) {
+    // rows listed, N >= 64: a running pointer per lane, reloaded when the lane passes to the next matrix (at most
+    // once per step; only for an element that exists, so only rows of the tile's matrices are read)
+    int j = 0, e = lane;
+    OT* p = out + row_of<REG>(rows, 0) * N + lane;
+    for (int t = lane; t < count; t += WAVE) {
+      if (e >= N) {
+        e -= N;
+        j++;
+        p = out + row_of<REG>(rows, j) * N + e;
+      }
+      *p = get(j, e);
+      e += WAVE;
+      p += WAVE;
+    }
   } else {
     int j = lane / N, e = lane - j * N;
     for (int t = lane; t < count; t += WAVE) {

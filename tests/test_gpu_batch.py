@@ -106,7 +106,8 @@ def test_every_strategy_gives_the_same_results():
     A = inputs(gs, 605)
     b = matfun_amd.Batch(gs)
     base = None
-    for mode, per_cu, order in [(0, 0, 0), (0, 0, 1), (0, 2, 1), (0, 6, 0), (1, 0, 0), (1, 2, 0), (1, 8, 0)]:
+    for mode, per_cu, order in [(0, 0, 0), (0, 0, 1), (0, 2, 1), (0, 6, 0), (1, 0, 0), (1, 2, 0), (1, 8, 0),
+                                (2, 0, 0), (2, 2, 0)]:
         print(f"strategy mode={mode} per_cu={per_cu} order={order}", flush=True)
         b.set_strategy(mode, per_cu, order)
         got = run_batch(b, A, None, False)
@@ -157,13 +158,15 @@ def test_perturbation_matches_cpu_build_and_keeps_symmetry():
     b.close()
 
 
-@pytest.mark.parametrize("mixed", [False, True])
-def test_graph_sequence_equals_eager_steps(mixed):
+@pytest.mark.parametrize("mixed,mode", [(False, 0), (True, 0), (False, 2)])
+def test_graph_sequence_equals_eager_steps(mixed, mode):
+    """mode 2 forks the class kernels onto the plan's streams: the capture follows the fork and the join"""
     import torch
     import matfun_amd
     gs = groups_all_classes(610)
     A0 = inputs(gs, 611)
     b = matfun_amd.Batch(gs)
+    b.set_strategy(mode, 0 if mode else 6, 1)
     prec = "mixed" if mixed else "fp64"
     iters, sigma, seed = 20, 1e-3, 42
     dA0 = torch.from_numpy(A0).cuda()
