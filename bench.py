@@ -259,6 +259,7 @@ def main():
     ap.add_argument("--mixed-gmf-n", type=int, default=1 << 22)
     ap.add_argument("--batch-elements", type=int, default=1 << 20)
     ap.add_argument("--batch-iters", type=int, default=20)
+    ap.add_argument("--no-tune", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU only; --gpus must be 1 (the batch shards trivially "
@@ -273,6 +274,13 @@ def main():
     torch.cuda.set_device(dev)
     lib = matfun_amd.library()
     n, ng = args.n, args.gmf_n
+    tiles = None
+    if not args.no_tune:
+        import matfun_amd.tune as T
+        log("bench: tuning the symmetric order-2 H store tile (matfun_amd.tune)")
+        tuned = T.tune_smf_tiles(layouts=("matrix",))
+        tiles = {f"{k[0]} {k[1]} {k[2]}": v[0] for k, v in tuned.items()}
+        log(f"bench: tiles {tiles}")
     log(f"bench: generating {n} symmetric and {ng} general inputs (seed 0)")
     A = W.spd(n, seed=0)
     dA = torch.from_numpy(A).to(dev)
@@ -388,7 +396,7 @@ def main():
            "roofline": {"bound": "hbm", "achieved": head["achieved_GBs"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
                         "frac": head["frac"], "traffic": profiled_traffic(n), "kernel": "mf::smf_kernel<KIND,true,2,double>",
                         "avg_kernel_ms": head["avg_kernel_ms"], "measured_copy_GBs": copy_gbs},
-           "legs": legs}
+           "legs": legs, "tuned_tiles": tiles}
     if not args.only_headline:
         out["mixed_vs_fp64"] = legs_c
         out["hetero_batch"] = legs_b

@@ -133,6 +133,16 @@ int mf_hip_sequence_create(const mf_hip_plan *plan, const double *A0, double *A,
 int mf_hip_sequence_launch(mf_hip_sequence *seq, void *stream);
 int mf_hip_sequence_destroy(mf_hip_sequence *seq);
 
+/* ---- tile choice of the order-2 symmetric kernels (no counterpart in the reference), per function (kind 0..3;
+ * 4 = pow with per-matrix exponents), layout (matrix_layout 0 Voigt, 1 matrix) and precision (mixed 0 FP64, 1 FP32
+ * outputs), for the whole process: 1 = 10 matrices per H store pass with the eigenvectors / second divided
+ * differences in their own LDS block; 10, 8, 5 = matrices per pass with that block sharing the first phase's LDS
+ * region (less LDS per wavefront: more wavefronts per compute unit; fewer matrices per pass: more lanes idle while H
+ * is assembled).  Results do not depend on it.  Defaults measured on MI355X; matfun_amd.tune picks them by timing.
+ * Returns 0 / -1 (set), the tile or -1 (get). */
+int smf_hip_set_tile(int kind, int matrix_layout, int mixed, int hpp);
+int smf_hip_get_tile(int kind, int matrix_layout, int mixed);
+
 /* library version ("0.1.3"-compatible API level plus this build); replaces smf_version (c/include/matfun/matfun.h:53) */
 const char *smf_hip_version(void);
 

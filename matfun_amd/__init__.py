@@ -47,6 +47,8 @@ def library():
         for fn in (lib.smf_hip_eval_pow, lib.smf_hip_eval_matrix_pow, lib.gmf_hip_eval_pow,
                    lib.smf_hip_eval_pow_mixed, lib.smf_hip_eval_matrix_pow_mixed, lib.gmf_hip_eval_pow_mixed):
             fn.argtypes, fn.restype = pargs, ctypes.c_int
+        lib.smf_hip_set_tile.argtypes = [ctypes.c_int] * 4
+        lib.smf_hip_get_tile.argtypes = [ctypes.c_int] * 3
         lib.smf_hip_version.restype = ctypes.c_char_p
         lib.smf_hip_target.restype = ctypes.c_char_p
         _lib = lib

@@ -74,7 +74,7 @@ MF_DEV void run_tile(const BatchArgs& a, const BatchTile& tl, const ListRows& ro
   if (FAM == 0) {
     constexpr int KIND = SLOT == 0 ? EXP : (SLOT == 1 ? LOG : (SLOT == 2 ? SQRT : POW));
     const double* c2 = opaque(g_tab.smf_c2[SLOT < 3 ? SLOT : 0]);
-    smf_tile<KIND, true, ORDER, OT, false, SLOT == SLOT_POW>(rows, tl.nm, a.A, F, J, H, SLOT == 2 ? 0.5 : 0.0,
+    smf_tile<KIND, true, ORDER, OT, 1, SLOT == SLOT_POW>(rows, tl.nm, a.A, F, J, H, SLOT == 2 ? 0.5 : 0.0,
                                                               a.alv, c2, lds, lutJ, lutH, lane);
   } else {
     constexpr int KIND = SLOT == 0 ? EXP : (SLOT == 1 ? LOG : POW);

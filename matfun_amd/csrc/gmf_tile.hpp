@@ -80,7 +80,7 @@ MF_DEV void gmf_tile(const R& rows, int nm, const double* __restrict__ A, OT* __
 #pragma unroll
   for (int k = 0; k < 9; k++) xr[lane * 9 + k] = (OT)o.F[k];
   __syncthreads();
-  copy_out<9, false, R_IN>(F, rows, 9 * nm, lane, [&](int j, int e) { return xr[9 * j + e]; });
+  copy_out<9, 1, R_IN>(F, rows, 9 * nm, lane, [&](int j, int e) { return xr[9 * j + e]; });
   if (ORDER >= 1) {
     __syncthreads();
 #pragma unroll
@@ -105,7 +105,7 @@ MF_DEV void gmf_tile(const R& rows, int nm, const double* __restrict__ A, OT* __
         for (int k = 0; k < 3; k++) xr[g * 81 + 27 * k + Lg] = out[k];
       }
       __syncthreads();
-      copy_out<81, false, R_J>(J, shift_rows(rows, p0), 81 * cnt, lane, [&](int j, int e) { return xr[81 * j + e]; });
+      copy_out<81, 1, R_J>(J, shift_rows(rows, p0), 81 * cnt, lane, [&](int j, int e) { return xr[81 * j + e]; });
       __syncthreads();
     }
     if (ORDER >= 2) {

@@ -193,5 +193,16 @@ extern "C" int gmf_hip_eval_pow_mixed(const double* alpha, int64_t n, const doub
   return gmf_pow_call(alpha, n, A, F, J, H, stream);
 }
 
+extern "C" int smf_hip_set_tile(int kind, int matrix_layout, int mixed, int hpp) {
+  if (kind < mf::EXP || kind > 4 || (hpp != 10 && hpp != 8 && hpp != 5 && hpp != mf::TILE_SEP10)) return -1;
+  mf::smf_tile_set(kind, matrix_layout != 0, mixed != 0, hpp);
+  return 0;
+}
+
+extern "C" int smf_hip_get_tile(int kind, int matrix_layout, int mixed) {
+  if (kind < mf::EXP || kind > 4) return -1;
+  return mf::smf_tile_get(kind, matrix_layout != 0, mixed != 0);
+}
+
 extern "C" const char* smf_hip_version(void) { return MATFUN_HIP_VERSION; }
 extern "C" const char* smf_hip_target(void) { return "gfx950"; }

@@ -29,6 +29,12 @@ void gmf_launch_pow(hipStream_t st, int64_t n, const double* alv, const double* 
 void gmf_launch_pow_mixed(hipStream_t st, int64_t n, const double* alv, const double* A, float* F, float* J, float* H,
                           const GmfCoef& recips);
 
+// the H store tile of the order-2 symmetric kernels per (kind 0..3 or 4 = per-matrix pow, layout, precision):
+// TILE_SEP10, or 10 / 8 / 5 matrices per pass with V / D2 sharing the phase-A LDS region
+constexpr int TILE_SEP10 = 1;
+int smf_tile_get(int slot, bool mat, bool mixed);
+void smf_tile_set(int slot, bool mat, bool mixed, int hpp);
+
 constexpr int64_t MAX_PER_LAUNCH = (int64_t)1 << 30;   // matrices per grid (work-items stay below 2^32)
 
 }  // namespace mf

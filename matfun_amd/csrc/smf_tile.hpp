@@ -1,9 +1,11 @@
 // smf_tile.hpp -- one tile (up to 64 symmetric 3x3 matrices, one wavefront) of the symmetric kernels:
 //   phase A, one matrix per lane: coalesced input through LDS, eigen-decomposition, F, the 21 distinct J values,
 //            the 10 second divided differences; F / J staged in LDS and written as one contiguous region per wave
-//   phase B, six lanes per matrix (one per Voigt row of H), ten matrices per pass: H rows from V and D2 staged in
-//            LDS, gathered into the API layout through a lookup table and written contiguously (a pass covers
-//            10 x 5.8 KB of consecutive output)
+//   phase B, six lanes per matrix (one per Voigt row of H), HPP matrices per pass (the LDS store tile: 10, 8 or 5):
+//            H rows from V and D2 staged in LDS, gathered into the API layout through a lookup table and written
+//            contiguously (a pass covers HPP x 5.8 KB of consecutive output).  SEP: V / D2 in their own LDS block,
+//            staged at the end of phase A; else staged after the F / J stores into the phase-A region (less LDS per
+//            wavefront, one more barrier, V / D2 held in registers across the stores)
 // Every store instruction of the wave writes 64 consecutive elements: the output (96 B .. 5.8 KB per matrix) bounds
 // the kernel by HBM write bandwidth, which needs whole cache lines per instruction.
 // OT = double: the FP64 form; OT = float: the mixed form (FP64 eigen-decomposition and divided differences, FP32
@@ -18,17 +20,24 @@ namespace mf {
 
 constexpr int SA = 27;             // phase-A tile per matrix: F (6) + Js (21)
 constexpr int SB = 19;             // V (9) + D2 (10)
-constexpr int HPP = 10;            // matrices per phase-B pass (6 lanes each)
 constexpr int SH = 127;            // staged H values per matrix (6 rows x 21, +1: odd stride)
+constexpr int HPP_DEFAULT = 10;    // matrices per phase-B pass (6 lanes each)
 
-// LDS of one tile, in bytes: region 1 (input block, phase-A tile, H tiles) and the V / D2 block
-template <bool MAT, int ORDER, class OT>
+// LDS of one tile, in bytes.  Not SEP: one region used in turn by the FP64 input block, the phase-A tile (F, Js),
+// then the V / D2 block (offset 0) with the H tile of HPP matrices after it (offset VB).  SEP: the V / D2 block at
+// offset VS, after a region used in turn by the input block, the phase-A tile and the H tile (offset 0).
+template <bool MAT, int ORDER, class OT, int HPP = HPP_DEFAULT, bool SEP = false>
 struct SmfLds {
   static constexpr int NIN = MAT ? 9 : 6;
-  static constexpr int R1 = (WAVE * SA > HPP * SH ? WAVE * SA : HPP * SH);
-  static constexpr int R1B = (R1 * (int)sizeof(OT) > WAVE * NIN * 8 ? R1 * (int)sizeof(OT) : WAVE * NIN * 8);
-  static constexpr int TBB = ORDER >= 2 ? WAVE * SB * (int)sizeof(OT) : 0;
-  static constexpr int BYTES = ((R1B + 15) / 16) * 16 + TBB;
+  static constexpr int IN = WAVE * NIN * 8;
+  static constexpr int TA = WAVE * SA * (int)sizeof(OT);
+  static constexpr int VBLK = ORDER >= 2 ? ((WAVE * SB * (int)sizeof(OT) + 15) / 16) * 16 : 0;
+  static constexpr int HT = ORDER >= 2 ? HPP * SH * (int)sizeof(OT) : 0;
+  static constexpr int VB = SEP ? 0 : VBLK;                               // H tile offset
+  static constexpr int TB = SEP ? HT : VBLK + HT;
+  static constexpr int R1 = IN > TA ? (IN > TB ? IN : TB) : (TA > TB ? TA : TB);
+  static constexpr int VS = SEP ? ((R1 + 15) / 16) * 16 : 0;             // V / D2 block offset
+  static constexpr int BYTES = SEP ? VS + VBLK : R1;
 };
 constexpr int smf_nj(bool mat) { return mat ? 81 : 36; }
 constexpr int smf_nh(bool mat) { return mat ? 729 : 216; }
@@ -51,18 +60,21 @@ MF_DEV void smf_luts(unsigned short* lutJ, unsigned short* lutH, int lane) {
     }
 }
 
-// one tile of nm <= 64 matrices; lds: SmfLds<MAT, ORDER, OT>::BYTES bytes, 16-byte aligned; the LUTs filled
-// (smf_luts) and visible to the wave; lane = threadIdx.x.  Ends with the tile's outputs written and a barrier (the
+// one tile of nm <= 64 matrices; lds: SmfLds<MAT, ORDER, OT, HPP, SEP>::BYTES bytes, 16-byte aligned; the LUTs
+// filled (smf_luts) and visible to the wave; lane = threadIdx.x; VEC: elements per lane and store of J and H
+// (copy_out).  Ends with the tile's outputs written and a barrier (the
 // LDS may be reused).
-template <int KIND, bool MAT, int ORDER, class OT, bool PAIRS, bool PA, class R>
+template <int KIND, bool MAT, int ORDER, class OT, int VEC, bool PA, class R, int HPP = HPP_DEFAULT,
+          bool SEP = false>
 MF_DEV void smf_tile(const R& rows, int nm, const double* __restrict__ x, OT* __restrict__ F, OT* __restrict__ J,
                      OT* __restrict__ H, double al, const double* __restrict__ alv, const double* __restrict__ c2,
                      unsigned char* lds, const unsigned short* lutJ, const unsigned short* lutH, int lane) {
-  using L = SmfLds<MAT, ORDER, OT>;
+  using L = SmfLds<MAT, ORDER, OT, HPP, SEP>;
   constexpr int NIN = L::NIN, NF = MAT ? 9 : 6, NJ = NF * NF, NH = NJ * NF;
   double* xin = reinterpret_cast<double*>(lds);
-  OT* t1 = reinterpret_cast<OT*>(lds);
-  OT* tb = reinterpret_cast<OT*>(lds + ((L::R1B + 15) / 16) * 16);
+  OT* t1 = reinterpret_cast<OT*>(lds);             // phase-A tile
+  OT* tb = reinterpret_cast<OT*>(lds + L::VS);     // V / D2
+  OT* th = reinterpret_cast<OT*>(lds + L::VB);     // H tile
 
   // ---- input, coalesced through LDS (lanes past the end take the identity)
   load_in<NIN>(xin, x, rows, nm, lane);
@@ -87,6 +99,7 @@ MF_DEV void smf_tile(const R& rows, int nm, const double* __restrict__ x, OT* __
   __syncthreads();
 
   // ---- phase A
+  OT vd[ORDER >= 2 ? SB : 1];                    // V, D2 of this lane's matrix, staged after the F / J stores
   {
     SmfLane<KIND, OT> s;
     double Fv[6];
@@ -102,38 +115,47 @@ MF_DEV void smf_tile(const R& rows, int nm, const double* __restrict__ x, OT* __
     if (ORDER >= 2) {
       s.template second_dd<PA>(al, c2);
 #pragma unroll
-      for (int k = 0; k < 9; k++) tb[lane * SB + k] = (OT)s.V[k / 3][k % 3];
+      for (int k = 0; k < 9; k++) vd[k] = (OT)s.V[k / 3][k % 3];
 #pragma unroll
-      for (int k = 0; k < 10; k++) tb[lane * SB + 9 + k] = (OT)(s.D2[k] * nanf);
+      for (int k = 0; k < 10; k++) vd[9 + k] = (OT)(s.D2[k] * nanf);
+      if (SEP) {
+#pragma unroll
+        for (int k = 0; k < SB; k++) tb[lane * SB + k] = vd[k];
+      }
     }
   }
   __syncthreads();
-  copy_out<NF, false, R_IN>(F, rows, NF * nm, lane, [&](int j, int e) { return t1[j * SA + f_src(MAT, e)]; });
+  copy_out<NF, 1, R_IN>(F, rows, NF * nm, lane, [&](int j, int e) { return t1[j * SA + f_src(MAT, e)]; });
   if (ORDER >= 1)
-    copy_out<NJ, PAIRS, R_J>(J, rows, NJ * nm, lane, [&](int j, int e) {
+    copy_out<NJ, VEC, R_J>(J, rows, NJ * nm, lane, [&](int j, int e) {
       const int u = lutJ[e];
       const OT val = t1[j * SA + 6 + (u & 255)];
       return (u >> 8) ? (OT)2 * val : val;
     });
   if (ORDER >= 2) {
+    if (!SEP) {
+      __syncthreads();                         // (the phase-A tile fully read)
+#pragma unroll
+      for (int k = 0; k < SB; k++) tb[lane * SB + k] = vd[k];
+    }
     // ---- phase B: lane = (matrix jj of the pass, Voigt row i)
     const int jj = lane / 6, ir = lane - 6 * jj;
     for (int p0 = 0; p0 < nm; p0 += HPP) {
       const int cnt = nm - p0 < HPP ? nm - p0 : HPP;
-      __syncthreads();                         // (previous tile fully read)
+      __syncthreads();                         // (V / D2 staged; the previous H tile fully read)
       if (jj < cnt) {
         const OT* sb = tb + (p0 + jj) * SB;
         const SmfHess<OT> hs(sb, sb + 9);
         OT h[21];
         hs.h_row(ir, h);
 #pragma unroll
-        for (int k = 0; k < 21; k++) t1[jj * SH + ir * 21 + k] = h[k];
+        for (int k = 0; k < 21; k++) th[jj * SH + ir * 21 + k] = h[k];
       }
       __syncthreads();
       const R sub = shift_rows(rows, p0);
-      copy_out<NH, PAIRS, R_H>(H, sub, NH * cnt, lane, [&](int j, int e) {
+      copy_out<NH, VEC, R_H>(H, sub, NH * cnt, lane, [&](int j, int e) {
         const int u = lutH[e];
-        const OT val = t1[j * SH + (u & 255)];
+        const OT val = th[j * SH + (u & 255)];
         return (u >> 8) == 0 ? val : ((u >> 8) == 1 ? (OT)0.5 * val : (OT)0.25 * val);
       });
     }

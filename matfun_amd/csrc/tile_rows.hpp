@@ -53,23 +53,34 @@ MF_DEV int64_t row_of(const R& r, int j) {
 }
 
 // out[row(j) N + e] = get(j, e) for the count = N nm elements (t = j N + e) of the tile's matrices j = 0 .. nm-1 in
-// region REG, by the wave.  RunRows: one contiguous block (out + base N + t); with PAIRS two consecutive elements per
-// lane and one vector store (16 B FP64 / 8 B FP32; the block must be aligned to it), else one element per lane.
-// ListRows: one element per lane, per-matrix row from LDS.
-template <int N, bool PAIRS, int REG, class R, class OT, class Get>
+// region REG, by the wave.  RunRows: one contiguous block (out + base N + t); with VEC > 1, VEC consecutive elements
+// per lane and one vector store (VEC x sizeof(OT) bytes: 16 B for FP64 pairs and FP32 quadruples; the block must be
+// aligned to it), the last count % VEC elements one at a time; else one element per lane.  ListRows: one element
+// per lane, per-matrix row from LDS.
+template <int N, int VEC, int REG, class R, class OT, class Get>
 MF_DEV void copy_out(OT* __restrict__ out, const R& rows, int count, int lane, Get get) {
-  if (R::RUN && PAIRS) {
-    typedef OT v2 __attribute__((ext_vector_type(2)));
+  if (R::RUN && VEC > 1) {
+    typedef OT vv __attribute__((ext_vector_type(VEC)));
     OT* blk = out + row_of<REG>(rows, 0) * N;
-    int t = 2 * lane, j = t / N, e = t - j * N;
-    for (; t + 1 < count; t += 2 * WAVE) {
-      const int e1 = e + 1 == N ? 0 : e + 1, j1 = e + 1 == N ? j + 1 : j;
-      const v2 val = {get(j, e), get(j1, e1)};
-      *reinterpret_cast<v2*>(blk + t) = val;
-      e += 2 * WAVE;
+    int t = VEC * lane, j = t / N, e = t - j * N;
+    for (; t + VEC - 1 < count; t += VEC * WAVE) {
+      vv val;
+      int jq = j, eq = e;
+#pragma unroll
+      for (int q = 0; q < VEC; q++) {
+        val[q] = get(jq, eq);
+        const bool wrap = eq + 1 == N;
+        eq = wrap ? 0 : eq + 1;
+        jq = wrap ? jq + 1 : jq;
+      }
+      *reinterpret_cast<vv*>(blk + t) = val;
+      e += VEC * WAVE;
       while (e >= N) { e -= N; j++; }
     }
-    if (t < count) blk[t] = get(j, e);         // (odd count: the last element)
+    for (; t < count; t++) {                     // (the tail: at most one lane, fewer than VEC elements)
+      blk[t] = get(j, e);
+      if (++e == N) { e = 0; j++; }
+    }
   } else if (R::RUN) {
     OT* blk = out + row_of<REG>(rows, 0) * N;
     int j = lane / N, e = lane - j * N;

@@ -1,0 +1,83 @@
+"""Autotuner of the symmetric order-2 kernels' H store tile: 1 (10 matrices per phase-B pass, eigenvectors and
+second divided differences in their own LDS block) or 10, 8, 5 (matrices per pass, that block sharing the first
+phase's LDS region).
+
+    import matfun_amd.tune
+    table = matfun_amd.tune.tune_smf_tiles()      # times every choice, keeps the fastest (smf_hip_set_tile)
+
+Per function (exp, log, sqrt, pow with one exponent, pow with per-matrix exponents), layout (matrix, Voigt) and
+precision (FP64, mixed): a smaller tile takes less LDS per wavefront, so more wavefronts share a compute unit, and
+leaves more lanes idle while H is assembled; which wins depends on the function's register use and output width.
+No result depends on the choice (checked bitwise by tests/test_gpu_tune.py).  The timing uses HIP events on torch's
+current stream over seeded SPD inputs resident on the device: per candidate the median of `reps` launches, the
+candidates interleaved.
+"""
+import numpy as np
+
+import matfun_amd
+
+TILES = (1, 10, 8, 5)
+SLOTS = {"exp": 0, "log": 1, "pow": 2, "sqrt": 3, "pow_alpha": 4}
+
+
+def _time_candidates(set_tile, fn, tiles, rounds):
+    """median over `rounds` of each candidate's time, the candidates taken in turn within a round (so that a drift
+    of the device's clocks or temperature affects them alike)"""
+    import torch
+    s = torch.cuda.current_stream()
+    ev = {t: [] for t in tiles}
+    for t in tiles:                                       # (warm-up, one launch each)
+        set_tile(t)
+        assert fn() == 0
+    torch.cuda.synchronize()
+    for _ in range(rounds):
+        for t in tiles:
+            set_tile(t)
+            e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            e0.record(s)
+            assert fn() == 0
+            e1.record(s)
+            ev[t].append((e0, e1))
+    torch.cuda.synchronize()
+    return {t: float(np.median([a.elapsed_time(b) for a, b in ev[t]])) for t in tiles}
+
+
+def tune_smf_tiles(funcs=tuple(SLOTS), layouts=("matrix", "voigt"), precisions=("fp64", "mixed"), n=1 << 21, reps=7,
+                   seed=0):
+    """times each tile choice, sets the fastest; returns {(func, layout, precision): (tile, {tile: ms})}"""
+    import torch
+    from matfun_amd import workloads as W
+    lib = matfun_amd.library()
+    A = W.spd(n, seed=seed, log10_range=(-2.0, np.log10(60.0)))   # (exp stays inside the FP32 range)
+    st = torch.cuda.current_stream().cuda_stream
+    out = {}
+    dal = torch.from_numpy(np.random.default_rng(seed).uniform(-1.0, 1.0, n)).cuda()
+    for layout in layouts:
+        X = torch.from_numpy(A if layout == "matrix" else W.voigt(A)).cuda()
+        nf = 9 if layout == "matrix" else 6
+        for prec in precisions:
+            dt = torch.float32 if prec == "mixed" else torch.float64
+            F = torch.empty(n * nf, dtype=dt, device="cuda")
+            J = torch.empty(n * nf ** 2, dtype=dt, device="cuda")
+            H = torch.empty(n * nf ** 3, dtype=dt, device="cuda")
+            p = (F.data_ptr(), J.data_ptr(), H.data_ptr())
+            mx = prec == "mixed"
+            for func in funcs:
+                slot = SLOTS[func]
+                if func == "pow_alpha":
+                    fn0 = {("matrix", False): lib.smf_hip_eval_matrix_pow, ("matrix", True): lib.smf_hip_eval_matrix_pow_mixed,
+                           ("voigt", False): lib.smf_hip_eval_pow, ("voigt", True): lib.smf_hip_eval_pow_mixed}[(layout, mx)]
+                    call = lambda: fn0(dal.data_ptr(), n, X.data_ptr(), *p, st)
+                else:
+                    fn0 = {("matrix", False): lib.smf_hip_eval_matrix, ("matrix", True): lib.smf_hip_eval_matrix_mixed,
+                           ("voigt", False): lib.smf_hip_eval, ("voigt", True): lib.smf_hip_eval_mixed}[(layout, mx)]
+                    al = 0.3 if func == "pow" else 0.0
+                    call = lambda: fn0(matfun_amd.KINDS[func], al, n, X.data_ptr(), *p, st)
+                ms = _time_candidates(lambda t: lib.smf_hip_set_tile(slot, layout == "matrix", mx, t), call, TILES,
+                                      reps)
+                best = min(ms, key=ms.get)
+                assert lib.smf_hip_set_tile(slot, layout == "matrix", mx, best) == 0
+                out[(func, layout, prec)] = (best, ms)
+            del F, J, H
+            torch.cuda.empty_cache()
+    return out
