@@ -76,8 +76,10 @@ MF_DEV double dfun(double x, double fx, double al) {
 // f[x, y, z] (smf_kernel.hpp:148-192): Taylor series around the middle point inside a cluster, else the difference
 // formula over the most distant pair.  PA (powers with a per-matrix exponent): the series coefficients
 // binom(al, k + 2) are formed in the loop from binom(al, 2) = al (al - 1) / 2 by c_k = c_(k-1) (al - k - 1) / (k + 2),
-// the recurrence of smf_coefs, and c is not read
-template <int KIND, bool PA = false>
+// the recurrence of smf_coefs, and c is not read.  NT: series terms (30 as the reference for FP64 results; the mixed
+// form's FP32 results take 20: the remainder is below 21 (1/4)^20 < 2e-11 of the sum for tau = 1/4, below
+// 3^20 / 20! < 2e-9 for the powers' tau = 3/|al| < 1/4, and below 1/22! for exp)
+template <int KIND, bool PA = false, int NT = SMF_NTERMS>
 MF_DEV double dd2(double li, double lj, double lk, double fi, double fj, double fk, double Dij, double Dik, double Djk,
                   double al, const double* __restrict__ c) {
   constexpr bool powlike = KIND == POW || KIND == SQRT;
@@ -116,7 +118,7 @@ MF_DEV double dd2(double li, double lj, double lk, double fi, double fj, double 
   if (cluster) {                       // h_k(u, w) = u h_(k-1) + w^k
     double h = 1.0, wk = 1.0, ck = PA ? al * (al - 1.0) * 0.5 : c[0], s = ck;
 #pragma unroll
-    for (int k = 1; k < SMF_NTERMS; k++) {
+    for (int k = 1; k < NT; k++) {
       wk = wk * w;
       h = u * h + wk;
       ck = PA ? ck * ((al - (k + 1.0)) * (1.0 / (k + 2))) : c[k];
@@ -270,7 +272,7 @@ struct SmfLane {
   }
 
   // ---- the 10 second divided differences (smf_kernel.hpp:330-338)
-  template <bool PA = false>
+  template <bool PA = false, int NT = SMF_NTERMS>
   MF_DEV void second_dd(double al, const double* __restrict__ c2) {
 #pragma unroll
     for (int i = 0; i < 3; i++)
@@ -279,7 +281,7 @@ struct SmfLane {
 #pragma unroll
         for (int k = j; k < 3; k++)
           D2[d2idx(i, j, k)] =
-              dd2<KIND, PA>(lam[i], lam[j], lam[k], fl[i], fl[j], fl[k], D[i][j], D[i][k], D[j][k], al, c2);
+              dd2<KIND, PA, NT>(lam[i], lam[j], lam[k], fl[i], fl[j], fl[k], D[i][j], D[i][k], D[j][k], al, c2);
   }
 };
 

@@ -113,7 +113,7 @@ MF_DEV void smf_tile(const R& rows, int nm, const double* __restrict__ x, OT* __
       for (int k = 0; k < 21; k++) t1[lane * SA + 6 + k] = Js[k] * (OT)nanf;
     }
     if (ORDER >= 2) {
-      s.template second_dd<PA>(al, c2);
+      s.template second_dd<PA, (sizeof(OT) == 4 ? 20 : SMF_NTERMS)>(al, c2);
 #pragma unroll
       for (int k = 0; k < 9; k++) vd[k] = (OT)s.V[k / 3][k % 3];
 #pragma unroll
