@@ -1,9 +1,11 @@
 // gmf_tile.hpp -- one tile (up to 64 general real 3x3 matrices, one wavefront) of the general kernels:
 //   phase 1, one matrix per lane: eigenvalues, divided differences, real basis, contracted tensors (gmf_device.hpp);
 //            r_m (27 values) and D3 (27) staged in LDS per matrix, D2 (9) and F (9) in a second region
-//   phase 2, lane groups: 27 lanes per matrix, two matrices per pass, assemble J (81: 3 per lane) and H (729: 27 per
-//            lane) into an LDS tile in the API layout, which the wave then writes as one contiguous block (2 x 5.8 KB
-//            of consecutive output per pass; whole cache lines per store instruction)
+//   phase 2, lane groups, assembling J and H into an LDS tile in the API layout that the wave then writes as one
+//            contiguous block: J with 9 lanes per matrix (9 values each), 7 matrices per pass (order 1) or 27 lanes
+//            per matrix (3 values each), 2 matrices per pass (order 2, where the smaller J tile keeps the LDS of a
+//            wavefront at 20.9 KB); H with 27 lanes per matrix (27 values each), 2 matrices per pass (2 x 5.8 KB of
+//            consecutive output per pass)
 // OT = double: the FP64 form; OT = float: the mixed form (FP64 per-lane math, FP32 staged tensors, J / H assembly
 // and outputs).  PA: powers with one exponent per matrix (alv[i], read by its lane; bt then holds 1 / j; a matrix
 // whose exponent is not finite gets NaN results).  Used by the homogeneous kernels (gmf_kernels.hip, RunRows) and the
@@ -16,14 +18,16 @@ namespace mf {
 
 // LDS layout (elements of OT):
 //   main [64][MS]   r1 (9), r2 (9); order 2: the 10 distinct D3 and sc2 (MS = 29; order 1: 19)
-//   x    [XS]       phase 1: F of the 64 matrices [0, 576), then D2 (6 distinct) and sc1 per matrix [162, 610);
-//                   J passes: the 2 x 81 output tile [0, 162); H passes: a half-block tile 2 x 378 (or 2 x 351)
+//   x    [XS]       phase 1: F of the 64 matrices [0, 576), then D2 (6 distinct) and sc1 per matrix [XAUX, +448);
+//                   J passes: the JMPP x 81 output tile [0, XAUX); H passes: a half-block tile 2 x 378 (or 2 x 351)
 // (the FP64 input block of 64 x 9 doubles overlays the start, before phase 1)
 template <int ORDER, class OT>
 struct GmfLds {
   static constexpr int MS = ORDER >= 2 ? 29 : 19;
-  static constexpr int XAUX = 162;
-  static constexpr int XS = ORDER == 0 ? 576 : (ORDER == 1 ? 610 : 756);
+  static constexpr int JG = ORDER == 1 ? 9 : 27;            // J: lanes per matrix
+  static constexpr int JMPP = WAVE / JG;                    // J: matrices per pass
+  static constexpr int XAUX = JMPP * 81;
+  static constexpr int XS = ORDER == 0 ? 576 : (XAUX + 448 > 756 ? XAUX + 448 : 756);
   static constexpr int MAIN = ORDER == 0 ? 0 : 64 * MS;
   static constexpr int TOT = MAIN + XS;
   static constexpr int BYTES = (TOT * (int)sizeof(OT) > WAVE * 9 * 8) ? TOT * (int)sizeof(OT) : WAVE * 9 * 8;
@@ -90,25 +94,37 @@ MF_DEV void gmf_tile(const R& rows, int nm, const double* __restrict__ A, OT* __
     xr[L::XAUX + lane * 7 + 6] = (OT)o.sc1;
     __syncthreads();
 
-    // ---- phase 2: lane (g, Lg) = matrix g of the pass, position Lg = 9 u + 3 v + w of a 27-lane group
-    constexpr int G = 27, MPP = WAVE / G;
-    const int g = lane / G, Lg = lane - g * G;
-    for (int p0 = 0; p0 < nm; p0 += MPP) {          // J: J_abcd for a = 0..2 at 27 a + (9b + 3c + d)
-      const int cnt = nm - p0 < MPP ? nm - p0 : MPP;
-      if (g < cnt) {
-        const int j = p0 + g;
+    // ---- phase 2, J: lane (gj, Lj) = matrix gj of the pass, Lj = 9 b + 3 c + d (27 lanes: J_abcd for a = 0..2) or
+    //      Lj = 3 c + d (9 lanes: J_abcd for a, b = 0..2), each value at 27 a + 9 b + 3 c + d of its matrix
+    constexpr int JG = L::JG, JMPP = L::JMPP;
+    const int gj = lane / JG, Lj = lane - gj * JG;
+    for (int p0 = 0; p0 < nm; p0 += JMPP) {
+      const int cnt = nm - p0 < JMPP ? nm - p0 : JMPP;
+      if (gj < cnt) {
+        const int j = p0 + gj;
         const OT* s = mainr + j * L::MS;
         const OT* aux = xr + L::XAUX + j * 7;
         OT out[3];
-        gmf_j_lane(s, s + 9, aux, aux[6], Lg / 9, (Lg / 3) % 3, Lg % 3, out);
+        if (JG == 27) {
+          gmf_j_lane(s, s + 9, aux, aux[6], Lj / 9, (Lj / 3) % 3, Lj % 3, out);
 #pragma unroll
-        for (int k = 0; k < 3; k++) xr[g * 81 + 27 * k + Lg] = out[k];
+          for (int k = 0; k < 3; k++) xr[gj * 81 + 27 * k + Lj] = out[k];
+        } else {
+#pragma unroll
+          for (int b = 0; b < 3; b++) {
+            gmf_j_lane(s, s + 9, aux, aux[6], b, Lj / 3, Lj % 3, out);
+#pragma unroll
+            for (int k = 0; k < 3; k++) xr[gj * 81 + 27 * k + 9 * b + Lj] = out[k];
+          }
+        }
       }
       __syncthreads();
       copy_out<81, 1, R_J>(J, shift_rows(rows, p0), 81 * cnt, lane, [&](int j, int e) { return xr[81 * j + e]; });
       __syncthreads();
     }
     if (ORDER >= 2) {
+      constexpr int G = 27, MPP = WAVE / G;            // H: lane (g, Lg), Lg = 9 d + 3 e + f
+      const int g = lane / G, Lg = lane - g * G;
       for (int p0 = 0; p0 < nm; p0 += MPP) {        // H: H_abcdef, (a,b,c) = 0..26 at 27 (9a + 3b + c) + (9d + 3e + f)
         const int cnt = nm - p0 < MPP ? nm - p0 : MPP;
         OT out[27];
