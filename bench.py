@@ -8,6 +8,8 @@ matrices (one smf_hip_eval_matrix call each), inputs resident in HBM, outputs ov
 Also measured (same JSON line, key "legs"): the mixed form of the same workload (FP32 outputs; log, sqrt, pow(0.3):
 exp of eigenvalues up to 1e2 is beyond the FP32 range) with its error against the FP64 form, and general matrices
 (gmf_hip_eval, n = 2^20 per function, similarity / complex-pair / shear / 4I+N mix, seed 0) in FP64 and mixed form.
+Also (key "legs"): the packed symmetric outputs of the headline matrices as Voigt vectors (smf_hip_eval_packed:
+F 6, J 21, H 126 values per matrix, 1272 B per matrix FP64) in both precisions.
 Key "mixed_vs_fp64": the mixed form against the FP64 form on smf log and smf pow with one exponent per matrix drawn
 uniform in [-1, 1] (order 2, n = 2^24, the headline distribution, seed 1) plus gmf log order 1 (n = 2^22, the general
 mix, seed 1): time per step of each form, the speed-up, and the mixed form's error against the FP64 form per output.
@@ -292,7 +294,7 @@ def main():
     stream = torch.cuda.current_stream()
     sh = stream.cuda_stream
 
-    def leg(name, fn, X, m, out_bytes, steps, warmup, funcs=FUNCS):
+    def leg(name, fn, X, m, out_bytes, steps, warmup, funcs=FUNCS, bpm=None):
         ev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in funcs]
         ptrs = (Fb.data_ptr(), Jb.data_ptr(), Hb.data_ptr())
 
@@ -316,7 +318,7 @@ def main():
             kern += [a.elapsed_time(b) for a, b in ev]
         wall = (time.perf_counter() - t0) / steps
         kms = float(np.mean(kern))
-        bpm = bytes_per_matrix(2, out_bytes)
+        bpm = bpm or bytes_per_matrix(2, out_bytes)
         gbs = m * bpm / (kms * 1e-3) / 1e9
         nf = len(funcs)
         per = {fname(k, al): m / (float(np.mean(kern[i::nf])) * 1e-3) for i, (k, al) in enumerate(funcs)}
@@ -353,6 +355,13 @@ def main():
     if not args.only_headline:
         legs["smf_mixed"] = leg("smf mixed", lib.smf_hip_eval_matrix_mixed, dA, n, 4, args.steps, args.warmup,
                                 funcs_mx)
+        # packed symmetric outputs (F 6, J 21, H 126 per matrix) of the same matrices as Voigt vectors
+        dV = torch.from_numpy(W.voigt(A)).to(dev)
+        legs["smf_packed_fp64"] = leg("smf packed fp64", lib.smf_hip_eval_packed, dV, n, 8, args.steps, args.warmup,
+                                      bpm=48 + 8 * 153)
+        legs["smf_packed_mixed"] = leg("smf packed mixed", lib.smf_hip_eval_packed_mixed, dV, n, 4, args.steps,
+                                       args.warmup, funcs_mx, bpm=48 + 4 * 153)
+        del dV
         legs["gmf_fp64"] = leg("gmf fp64", lib.gmf_hip_eval, dG, ng, 8, args.steps, args.warmup)
         legs["gmf_mixed"] = leg("gmf mixed", lib.gmf_hip_eval_mixed, dG, ng, 4, args.steps, args.warmup)
         legs["smf_mixed"]["max_err_vs_fp64"] = mixed_error(lib.smf_hip_eval_matrix, lib.smf_hip_eval_matrix_mixed,

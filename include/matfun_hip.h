@@ -68,6 +68,22 @@ int smf_hip_eval_matrix_pow_mixed(const double *alpha, int64_t n, const double *
 int gmf_hip_eval_pow_mixed(const double *alpha, int64_t n, const double *A, float *F, float *J, float *H,
                            void *stream);
 
+/* ---- packed symmetric outputs (no counterpart in the reference API; the H packing is the reference kernel's own
+ * HPACK layout, c/include/matfun/smf_kernel.hpp:196-199): Voigt input v (n,6) as smf_hip_eval, outputs of the
+ * distinct values only --
+ *   F (n,6)    the Voigt vector of f(A)
+ *   J (n,21)   Jp[tri(i,k)] = dF_ab / dA_cd, i = (a,b), k = (c,d) Voigt indices, i <= k (symmetric in i <-> k)
+ *   H (n,126)  Hp[21 i + tri(p,q)] = d2 F_i / dv_p dv_q, p <= q (symmetric in p <-> q)
+ * with tri(p,q) = p (11 - p) / 2 + q the row-major upper triangle of 6x6.  1224 instead of 6624 bytes per matrix at
+ * order 2 (FP64).  The API layouts follow with exact weights: Voigt J[6i+k] = Jp[tri(min,max)] w(k), w = 1 (k < 3)
+ * or 2; Voigt H[(6i+p)6+q] = Hp[21 i + tri(min(p,q), max(p,q))]; matrix layout J[ab][cd] = Jp[tri(i,k)],
+ * H[ab][cd][ef] = Hp[21 i + tri(k,l)] u(cd) u(ef), u = 1 (diagonal) or 1/2 -- bitwise the results of
+ * smf_hip_eval / smf_hip_eval_matrix.  Same arguments and return codes as smf_hip_eval. */
+int smf_hip_eval_packed(int kind, double alpha, int64_t n, const double *v, double *F, double *J, double *H,
+                        void *stream);
+int smf_hip_eval_packed_mixed(int kind, double alpha, int64_t n, const double *v, float *F, float *J, float *H,
+                              void *stream);
+
 /* ---- heterogeneous batches (no counterpart in the reference API, which evaluates one function per call): one
  * persistent kernel evaluates groups of matrices with different functions and orders -- e.g. the integration points
  * of finite elements, each element with its own constitutive function.  Groups follow one another: the matrices of

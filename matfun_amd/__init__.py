@@ -19,7 +19,8 @@ import math
 import os
 
 __version__ = "0.1.3"
-__all__ = ["smf", "gmf", "Batch", "Group", "library", "library_path", "backends", "KINDS"]
+__all__ = ["smf", "smf_packed", "expand_packed", "gmf", "Batch", "Group", "library", "library_path", "backends",
+           "KINDS"]
 
 KINDS = {"exp": 0, "log": 1, "pow": 2, "sqrt": 3}
 _HERE = os.path.dirname(os.path.abspath(__file__))
@@ -42,6 +43,8 @@ def library():
         args = [ctypes.c_int, ctypes.c_double, ctypes.c_int64] + [ctypes.c_void_p] * 5
         for fn in (lib.smf_hip_eval, lib.smf_hip_eval_matrix, lib.gmf_hip_eval, lib.smf_hip_eval_mixed,
                    lib.smf_hip_eval_matrix_mixed, lib.gmf_hip_eval_mixed):
+            fn.argtypes, fn.restype = args, ctypes.c_int
+        for fn in (lib.smf_hip_eval_packed, lib.smf_hip_eval_packed_mixed):
             fn.argtypes, fn.restype = args, ctypes.c_int
         pargs = [ctypes.c_void_p, ctypes.c_int64] + [ctypes.c_void_p] * 5
         for fn in (lib.smf_hip_eval_pow, lib.smf_hip_eval_matrix_pow, lib.gmf_hip_eval_pow,
@@ -134,6 +137,61 @@ def smf(kind, A, alpha=None, order=0, precision="fp64"):
               (lib.smf_hip_eval_mixed if mixed else lib.smf_hip_eval))
         return _run(fn, kind, alpha, A, shape[:-1], ((6,), (6, 6), (6, 6, 6)), order, dt, pa)
     raise ValueError(f"expected matrices (..., 3, 3) or Voigt vectors (..., 6), got shape {shape}")
+
+
+def smf_packed(kind, v, alpha=None, order=0, precision="fp64"):
+    """f(A) and its derivatives for symmetric matrices given as Voigt vectors v (..., 6), as the distinct values only:
+    F (..., 6) Voigt, J (..., 21) = dF_ab/dA_cd for Voigt indices i = (a,b) <= k = (c,d), H (..., 126) = the Voigt
+    second derivatives d2F_i/dv_p dv_q for p <= q, row i at 21 i (smf_hip_eval_packed).  expand_packed gives the
+    Voigt or matrix layouts of smf."""
+    _prepare(kind, alpha, order)
+    mixed, dt = _precision(precision)
+    lib = library()
+    shape = tuple(v.shape)
+    if shape[-1:] != (6,):
+        raise ValueError(f"expected Voigt vectors (..., 6), got shape {shape}")
+    return _run(lib.smf_hip_eval_packed_mixed if mixed else lib.smf_hip_eval_packed, kind, alpha, v, shape[:-1],
+                ((6,), (21,), (126,)), order, dt)
+
+
+def _tri6():
+    import numpy as np
+    T = np.zeros((6, 6), dtype=np.int64)
+    for p in range(6):
+        for q in range(6):
+            a, b = min(p, q), max(p, q)
+            T[p, q] = a * (11 - a) // 2 + b
+    return T
+
+
+def expand_packed(F, J=None, H=None, layout="voigt"):
+    """the Voigt (F 6, J 6x6, H 6x6x6) or matrix (F 3x3, J 3^4, H 3^6) layout of packed results (NumPy arrays,
+    any leading shape); the weights are exact (1, 2 / 1, 1/2), so the results are those of smf / smf_hip_eval*"""
+    import numpy as np
+    F = np.asarray(F)
+    T = _tri6()
+    lead = F.shape[:-1]
+    if layout == "voigt":
+        w = np.array([1.0, 1.0, 1.0, 2.0, 2.0, 2.0])
+        out = [F]
+        if J is not None:
+            out.append(np.asarray(J)[..., T] * w)
+        if H is not None:
+            out.append(np.asarray(H)[..., 21 * np.arange(6)[:, None, None] + T[None, :, :]])
+        return tuple(out) if len(out) > 1 else out[0]
+    if layout != "matrix":
+        raise ValueError("layout must be 'voigt' or 'matrix'")
+    vm = np.array([[0, 3, 4], [3, 1, 5], [4, 5, 2]])
+    u = np.where(np.eye(3, dtype=bool), 1.0, 0.5)
+    out = [F[..., vm]]
+    if J is not None:
+        out.append(np.asarray(J)[..., T[vm[:, :, None, None], vm[None, None, :, :]]])
+    if H is not None:
+        idx = 21 * vm[:, :, None, None, None, None] + T[vm[None, None, :, :, None, None], vm[None, None, None, None, :, :]]
+        wt = u[:, :, None, None] * u[None, None, :, :]
+        out.append(np.asarray(H)[..., idx] * wt)
+    del lead
+    return tuple(out) if len(out) > 1 else out[0]
 
 
 def gmf(kind, A, alpha=None, order=0, precision="fp64"):

@@ -99,6 +99,33 @@ int gmf_call(int kind, double alpha, int64_t n, const double* A, OT* F, OT* J, O
   return 0;
 }
 
+void packed_chunk(hipStream_t st, int kind, int64_t n, const double* x, double* F, double* J, double* H, double al,
+                  const mf::SmfCoef& c2) {
+  mf::smf_launch_packed(st, kind, n, x, F, J, H, al, c2);
+}
+void packed_chunk(hipStream_t st, int kind, int64_t n, const double* x, float* F, float* J, float* H, double al,
+                  const mf::SmfCoef& c2) {
+  mf::smf_launch_packed_mixed(st, kind, n, x, F, J, H, al, c2);
+}
+
+// symmetric matrices in Voigt form, packed outputs: F 6, J 21, H 126 values per matrix
+template <class OT>
+int smf_packed_call(int kind, double alpha, int64_t n, const double* v, OT* F, OT* J, OT* H, void* stream) {
+  if (!valid(kind, alpha, n, J, H)) return -1;
+  if (kind != mf::POW) alpha = kind == mf::SQRT ? 0.5 : 0.0;
+  if (n == 0) return 0;
+  if (n > SMF_MAX_N) return -2;
+  mf::SmfCoef c2;
+  mf::smf_coefs(kind, alpha, c2);
+  for (int64_t lo = 0; lo < n; lo += mf::MAX_PER_LAUNCH) {
+    const int64_t m = n - lo < mf::MAX_PER_LAUNCH ? n - lo : mf::MAX_PER_LAUNCH;
+    packed_chunk((hipStream_t)stream, kind, m, v + 6 * lo, F + 6 * lo, J ? J + 21 * lo : nullptr,
+                 H ? H + 126 * lo : nullptr, alpha, c2);
+    if (launched()) return -2;
+  }
+  return 0;
+}
+
 // powers with one exponent per matrix (device array alpha of n doubles, any finite value)
 template <class OT>
 int smf_pow_call(const double* alpha, int64_t n, const double* x, bool mat, OT* F, OT* J, OT* H, void* stream) {
@@ -191,6 +218,16 @@ extern "C" int smf_hip_eval_matrix_pow_mixed(const double* alpha, int64_t n, con
 extern "C" int gmf_hip_eval_pow_mixed(const double* alpha, int64_t n, const double* A, float* F, float* J, float* H,
                                       void* stream) {
   return gmf_pow_call(alpha, n, A, F, J, H, stream);
+}
+
+extern "C" int smf_hip_eval_packed(int kind, double alpha, int64_t n, const double* v, double* F, double* J,
+                                   double* H, void* stream) {
+  return smf_packed_call(kind, alpha, n, v, F, J, H, stream);
+}
+
+extern "C" int smf_hip_eval_packed_mixed(int kind, double alpha, int64_t n, const double* v, float* F, float* J,
+                                         float* H, void* stream) {
+  return smf_packed_call(kind, alpha, n, v, F, J, H, stream);
 }
 
 extern "C" int smf_hip_set_tile(int kind, int matrix_layout, int mixed, int hpp) {

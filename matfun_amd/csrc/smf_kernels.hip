@@ -19,18 +19,20 @@ static std::atomic<int> g_tile[5][2][2] = {
 int smf_tile_get(int slot, bool mat, bool mixed) { return g_tile[slot][mat][mixed].load(std::memory_order_relaxed); }
 void smf_tile_set(int slot, bool mat, bool mixed, int hpp) { g_tile[slot][mat][mixed].store(hpp, std::memory_order_relaxed); }
 
-template <int KIND, bool MAT, int ORDER, class OT, int VEC, bool PA, int HPP = HPP_DEFAULT, bool SEP = true>
+template <int KIND, bool MAT, int ORDER, class OT, int VEC, bool PA, int HPP = HPP_DEFAULT, bool SEP = true,
+          bool PACK = false>
 __global__ void __launch_bounds__(WAVE) smf_kernel(int64_t n, const double* __restrict__ x, OT* __restrict__ F,
                                                    OT* __restrict__ J, OT* __restrict__ H, double al,
                                                    const double* __restrict__ alv, SmfCoef c2) {
   using L = SmfLds<MAT, ORDER, OT, HPP, SEP>;
+  constexpr bool LUT = !PACK;
   __shared__ __attribute__((aligned(16))) unsigned char lds[L::BYTES];
-  __shared__ unsigned short lutJ[ORDER >= 1 ? smf_nj(MAT) : 1], lutH[ORDER >= 2 ? smf_nh(MAT) : 1];
+  __shared__ unsigned short lutJ[LUT && ORDER >= 1 ? smf_nj(MAT) : 1], lutH[LUT && ORDER >= 2 ? smf_nh(MAT) : 1];
   const int64_t base = (int64_t)blockIdx.x * WAVE;
   const int nm = (int)(n - base < WAVE ? n - base : WAVE);
-  smf_luts<MAT, ORDER>(lutJ, lutH, threadIdx.x);
-  smf_tile<KIND, MAT, ORDER, OT, VEC, PA, RunRows, HPP, SEP>(RunRows{base}, nm, x, F, J, H, al, alv, c2.c, lds, lutJ,
-                                                             lutH, threadIdx.x);
+  if (LUT) smf_luts<MAT, ORDER>(lutJ, lutH, threadIdx.x);
+  smf_tile<KIND, MAT, ORDER, OT, VEC, PA, RunRows, HPP, SEP, PACK>(RunRows{base}, nm, x, F, J, H, al, alv, c2.c, lds,
+                                                                   lutJ, lutH, threadIdx.x);
 }
 
 #define MF_LAUNCH(ORD, V, HP, SP)                                                                                    \
@@ -96,6 +98,48 @@ static void launch_pow(hipStream_t st, bool mat, int64_t n, const double* alv, c
   const SmfCoef unused{};
   if (mat) launch_kind<POW, true, OT, true>(st, blocks, n, x, F, J, H, 1.0, alv, unused);
   else launch_kind<POW, false, OT, true>(st, blocks, n, x, F, J, H, 1.0, alv, unused);
+}
+
+// packed outputs (Voigt input; F 6, J 21, H 126 per matrix): the default tile (10 matrices per H pass, whose
+// 1260-element pass and the 1344-element J block of a wave keep 16-byte alignment for FP32 quadruples too)
+template <int KIND, class OT>
+static void launch_packed_kind(hipStream_t st, unsigned blocks, int64_t n, const double* x, OT* F, OT* J, OT* H,
+                               double al, const SmfCoef& c2) {
+  auto aligned = [&](uintptr_t a) { return (!J || ((uintptr_t)J & (a - 1)) == 0) && (!H || ((uintptr_t)H & (a - 1)) == 0); };
+  constexpr int V2 = sizeof(OT) == 4 ? 4 : 2;
+  const double* alv = nullptr;
+#define MF_PK(ORD, V)                                                                                               \
+  hipLaunchKernelGGL((smf_kernel<KIND, false, ORD, OT, V, false, HPP_DEFAULT, true, true>), dim3(blocks), dim3(WAVE), \
+                     0, st, n, x, F, J, H, al, alv, c2)
+  if (!J) MF_PK(0, 1);
+  else if (!H) {
+    if (aligned(16)) MF_PK(1, V2); else if (aligned(2 * sizeof(OT))) MF_PK(1, 2); else MF_PK(1, 1);
+  } else {
+    if (aligned(16)) MF_PK(2, V2); else if (aligned(2 * sizeof(OT))) MF_PK(2, 2); else MF_PK(2, 1);
+  }
+#undef MF_PK
+}
+
+template <class OT>
+static void launch_packed(hipStream_t st, int kind, int64_t n, const double* x, OT* F, OT* J, OT* H, double al,
+                          const SmfCoef& c2) {
+  const unsigned blocks = (unsigned)((n + WAVE - 1) / WAVE);
+  switch (kind) {
+    case EXP: launch_packed_kind<EXP, OT>(st, blocks, n, x, F, J, H, al, c2); break;
+    case LOG: launch_packed_kind<LOG, OT>(st, blocks, n, x, F, J, H, al, c2); break;
+    case SQRT: launch_packed_kind<SQRT, OT>(st, blocks, n, x, F, J, H, al, c2); break;
+    default: launch_packed_kind<POW, OT>(st, blocks, n, x, F, J, H, al, c2); break;
+  }
+}
+
+void smf_launch_packed(hipStream_t st, int kind, int64_t n, const double* x, double* F, double* J, double* H,
+                       double al, const SmfCoef& c2) {
+  launch_packed<double>(st, kind, n, x, F, J, H, al, c2);
+}
+
+void smf_launch_packed_mixed(hipStream_t st, int kind, int64_t n, const double* x, float* F, float* J, float* H,
+                             double al, const SmfCoef& c2) {
+  launch_packed<float>(st, kind, n, x, F, J, H, al, c2);
 }
 
 void smf_launch(hipStream_t st, int kind, bool mat, int64_t n, const double* x, double* F, double* J, double* H,

@@ -19,6 +19,11 @@ void smf_launch_mixed(hipStream_t st, int kind, bool mat, int64_t n, const doubl
 void gmf_launch_mixed(hipStream_t st, int kind, int64_t n, const double* A, float* F, float* J, float* H, double al,
                       const GmfCoef& bt);
 
+// packed outputs (Voigt input): F 6, J 21, H 126 per matrix
+void smf_launch_packed(hipStream_t st, int kind, int64_t n, const double* x, double* F, double* J, double* H,
+                       double al, const SmfCoef& c2);
+void smf_launch_packed_mixed(hipStream_t st, int kind, int64_t n, const double* x, float* F, float* J, float* H,
+                             double al, const SmfCoef& c2);
 // powers with one exponent per matrix (alv: n device doubles); gmf: recips from gmf_recips
 void smf_launch_pow(hipStream_t st, bool mat, int64_t n, const double* alv, const double* x, double* F, double* J,
                     double* H);

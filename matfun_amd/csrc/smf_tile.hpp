@@ -10,7 +10,9 @@
 // the kernel by HBM write bandwidth, which needs whole cache lines per instruction.
 // OT = double: the FP64 form; OT = float: the mixed form (FP64 eigen-decomposition and divided differences, FP32
 // derivative assembly and FP32 outputs).  PA: powers with one exponent per matrix (alv[i], read by its lane; a
-// matrix whose exponent is not finite gets NaN results).  Used by the homogeneous kernels (smf_kernels.hip, RunRows)
+// matrix whose exponent is not finite gets NaN results).  PACK (Voigt input): the packed outputs -- F the 6 Voigt
+// values, J the 21 distinct values of dF_ab/dA_cd (Js), H the 126 distinct Voigt second derivatives (6 rows of 21) --
+// written as they are staged, without the layout tables.  Used by the homogeneous kernels (smf_kernels.hip, RunRows)
 // and the heterogeneous batch kernel (batch.hip, ListRows).
 #pragma once
 #include "smf_device.hpp"
@@ -65,12 +67,13 @@ MF_DEV void smf_luts(unsigned short* lutJ, unsigned short* lutH, int lane) {
 // (copy_out).  Ends with the tile's outputs written and a barrier (the
 // LDS may be reused).
 template <int KIND, bool MAT, int ORDER, class OT, int VEC, bool PA, class R, int HPP = HPP_DEFAULT,
-          bool SEP = false>
+          bool SEP = false, bool PACK = false>
 MF_DEV void smf_tile(const R& rows, int nm, const double* __restrict__ x, OT* __restrict__ F, OT* __restrict__ J,
                      OT* __restrict__ H, double al, const double* __restrict__ alv, const double* __restrict__ c2,
                      unsigned char* lds, const unsigned short* lutJ, const unsigned short* lutH, int lane) {
   using L = SmfLds<MAT, ORDER, OT, HPP, SEP>;
-  constexpr int NIN = L::NIN, NF = MAT ? 9 : 6, NJ = NF * NF, NH = NJ * NF;
+  static_assert(!(PACK && MAT), "packed outputs take Voigt input");
+  constexpr int NIN = L::NIN, NF = MAT ? 9 : 6, NJ = PACK ? 21 : NF * NF, NH = PACK ? 126 : NF * NF * NF;
   double* xin = reinterpret_cast<double*>(lds);
   OT* t1 = reinterpret_cast<OT*>(lds);             // phase-A tile
   OT* tb = reinterpret_cast<OT*>(lds + L::VS);     // V / D2
@@ -128,6 +131,7 @@ MF_DEV void smf_tile(const R& rows, int nm, const double* __restrict__ x, OT* __
   copy_out<NF, 1, R_IN>(F, rows, NF * nm, lane, [&](int j, int e) { return t1[j * SA + f_src(MAT, e)]; });
   if (ORDER >= 1)
     copy_out<NJ, VEC, R_J>(J, rows, NJ * nm, lane, [&](int j, int e) {
+      if (PACK) return t1[j * SA + 6 + e];
       const int u = lutJ[e];
       const OT val = t1[j * SA + 6 + (u & 255)];
       return (u >> 8) ? (OT)2 * val : val;
@@ -154,6 +158,7 @@ MF_DEV void smf_tile(const R& rows, int nm, const double* __restrict__ x, OT* __
       __syncthreads();
       const R sub = shift_rows(rows, p0);
       copy_out<NH, VEC, R_H>(H, sub, NH * cnt, lane, [&](int j, int e) {
+        if (PACK) return th[j * SH + e];
         const int u = lutH[e];
         const OT val = th[j * SH + (u & 255)];
         return (u >> 8) == 0 ? val : ((u >> 8) == 1 ? (OT)0.5 * val : (OT)0.25 * val);
