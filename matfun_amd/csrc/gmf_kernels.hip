@@ -5,8 +5,11 @@
 
 namespace mf {
 
+// FP32 outputs at order 1: a register budget of four wavefronts per SIMD (128 VGPRs; the LDS of 8.9 KB per
+// wavefront allows 18 per CU)
 template <int KIND, int ORDER, class OT, bool PA>
-__global__ void __launch_bounds__(WAVE) gmf_kernel(int64_t n, const double* __restrict__ A, OT* __restrict__ F,
+__global__ void __launch_bounds__(WAVE) __attribute__((amdgpu_waves_per_eu((ORDER == 1 && sizeof(OT) == 4) ? 4 : 1)))
+gmf_kernel(int64_t n, const double* __restrict__ A, OT* __restrict__ F,
                                                    OT* __restrict__ J, OT* __restrict__ H, double al,
                                                    const double* __restrict__ alv, GmfCoef bt) {
   __shared__ __attribute__((aligned(16))) unsigned char lds[GmfLds<ORDER, OT>::BYTES];
