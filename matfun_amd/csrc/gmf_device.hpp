@@ -158,10 +158,24 @@ MF_DEV cpx node_value(double al, double sig, double m, cpx x, cpx mu, int k) {
   return out;
 }
 
-// number of group-series terms (gmf_kernel.hpp:73-97)
+// 1 / j for j = 1 .. GMF_KMAX (entry 0: 1), computed at compile time (correctly rounded quotients)
+struct GmfRecip {
+  double r[GMF_KMAX + 1];
+};
+constexpr GmfRecip gmf_recip_table() {
+  GmfRecip t{};
+  t.r[0] = 1.0;
+  for (int j = 1; j <= GMF_KMAX; j++) t.r[j] = 1.0 / j;
+  return t;
+}
+
+// number of group-series terms (gmf_kernel.hpp:73-97); the bound's binomial C(k + n, n) advances by a product with
+// 1 / (k + 1) from rinv (gmf_recip_table) instead of a division (the count can differ from the reference's only for a
+// term within rounding of the 1e-17 cut, which changes the sum by less than that)
 // PA: b holds 1 / j and the coefficients binom(al, j) are formed on the way (gmf_recips)
 template <bool PA = false>
-MF_DEV int series_terms(const double* __restrict__ b, double al, int n, double r, double q) {
+MF_DEV int series_terms(const double* __restrict__ b, double al, int n, double r, double q,
+                        const double* __restrict__ rinv) {
   int M = 1;
   if (r > 0) {
     double best = 0.0, c = 1.0, rk = 1.0, bn = 1.0;
@@ -172,7 +186,7 @@ MF_DEV int series_terms(const double* __restrict__ b, double al, int n, double r
       best = t > best ? t : best;
       M = k;
       if ((k > 2 && t <= 1e-17 * best) || !(rk > 0)) break;
-      c = c * (k + n + 1) / (k + 1);
+      c = c * (k + n + 1) * rinv[k + 1];
       rk *= r;
       if (PA) bn = bn * ((al - (n + k)) * b[n + k + 1 < GMF_KMAX ? n + k + 1 : GMF_KMAX - 1]);
     }
@@ -198,8 +212,10 @@ struct GmfOut {
 };
 
 // PA (powers with a per-matrix exponent al): bt holds 1 / j instead of the coefficients b_j (gmf_recips)
+// rinv: gmf_recip_table()
 template <int KIND, int ORDER, bool PA = false>
-MF_DEV void gmf_lane(const double* a, double al, const double* __restrict__ bt, GmfOut& o) {
+MF_DEV void gmf_lane(const double* a, double al, const double* __restrict__ bt, GmfOut& o,
+                     const double* __restrict__ rinv) {
   constexpr int ND = ORDER == 0 ? 3 : (ORDER == 1 ? 9 : 19);
   const double RHO = 0.7, R_EXP = 4.0, RHO_POLE = 0.6;
   double amax = fabs(a[0]);
@@ -430,7 +446,7 @@ MF_DEV void gmf_lane(const double* a, double al, const double* __restrict__ bt, 
 #pragma unroll
   for (int c = 0; c < ND; c++) D[c] = mk(0.0, 0.0);
   if (hasg) {
-    const int K = series_terms<PA>(bt, al, 3 * (ORDER + 1) - 1, rr, rq);
+    const int K = series_terms<PA>(bt, al, 3 * (ORDER + 1) - 1, rr, rq, rinv);
     const double b0 = KIND == LOG ? node_value<LOG>(al, sig, m, mk(cxg, 0.0), mk(cmu, 0.0), 0).re : 1.0;
     cpx E[ND], acc[ND];
 #pragma unroll

@@ -16,6 +16,9 @@
 
 namespace mf {
 
+// the reciprocals of the series-length bound, in constant memory (uniform reads: scalar loads)
+static __constant__ GmfRecip g_gmf_recip = gmf_recip_table();
+
 // LDS layout (elements of OT):
 //   main [64][MS]   r1 (9), r2 (9); order 2: the 10 distinct D3 and sc2 (MS = 29; order 1: 19)
 //   x    [XS]       phase 1: F of the 64 matrices [0, 576), then D2 (6 distinct) and sc1 per matrix [XAUX, +448);
@@ -59,7 +62,7 @@ MF_DEV void gmf_tile(const R& rows, int nm, const double* __restrict__ A, OT* __
 
   // ---- phase 1
   GmfOut o;
-  gmf_lane<KIND, ORDER, PA>(a, al, bt, o);
+  gmf_lane<KIND, ORDER, PA>(a, al, bt, o, g_gmf_recip.r);
   if (PA) {                                          // (J and H carry the factors sc1, sc2)
     const double nanf = alok ? 1.0 : dnan();
 #pragma unroll

@@ -65,7 +65,8 @@ void gmf_all(double al, const double* alv, int64_t n, const double* A, double* F
     double nanf = 1.0;
     if (PA) al = exponent_of(alv, m, nanf);
     mf::GmfOut o;
-    mf::gmf_lane<KIND, ORDER, PA>(A + 9 * m, al, bt.b, o);
+    static constexpr mf::GmfRecip rinv = mf::gmf_recip_table();
+    mf::gmf_lane<KIND, ORDER, PA>(A + 9 * m, al, bt.b, o, rinv.r);
     for (int k = 0; k < 9; k++) o.F[k] *= nanf;
     o.sc1 *= nanf;
     o.sc2 *= nanf;
