@@ -68,7 +68,7 @@ static __constant__ BatchTables g_tab;
 // one tile of class CLS (lane and the table pointers opaque per tile)
 template <int CLS, class OT>
 MF_DEV void run_tile(const BatchArgs& a, const BatchTile& tl, const ListRows& rows, OT* F, OT* J, OT* H,
-                     unsigned char* lds, const unsigned short* lutJ, const unsigned short* lutH) {
+                     unsigned char* lds, const short* lutJ, const short* lutH) {
   constexpr int FAM = cls_fam(CLS), SLOT = cls_slot(CLS), ORDER = cls_order(CLS);
   const int lane = opaque((int)threadIdx.x);
   if (FAM == 0) {
@@ -95,7 +95,7 @@ MF_DEV void stage_rows(const BatchArgs& a, const BatchTile& tl, int64_t* rows, i
 
 template <int C, class OT>
 MF_DEV void dispatch_from(int cls, const BatchArgs& a, const BatchTile& tl, const ListRows& rows, OT* F, OT* J, OT* H,
-                          unsigned char* lds, const unsigned short* lutJ, const unsigned short* lutH) {
+                          unsigned char* lds, const short* lutJ, const short* lutH) {
   if constexpr (C < NCLS) {
     if (cls == C) run_tile<C, OT>(a, tl, rows, F, J, H, lds, lutJ, lutH);
     else dispatch_from<C + 1, OT>(cls, a, tl, rows, F, J, H, lds, lutJ, lutH);
@@ -113,10 +113,10 @@ template <class OT, bool LOOP>
 __global__ void __launch_bounds__(WAVE) __attribute__((amdgpu_waves_per_eu(2)))
 batch_kernel(BatchArgs a, OT* __restrict__ F, OT* __restrict__ J, OT* __restrict__ H) {
   __shared__ __attribute__((aligned(16))) unsigned char lds[batch_lds_bytes<OT>()];
-  __shared__ unsigned short lutJ[81], lutH[729];
+  __shared__ short lutJ[81 + LUT_PAD], lutH[729 + LUT_PAD];
   __shared__ int64_t rows[3 * WAVE];
   const int lane = threadIdx.x;
-  smf_luts<true, 2>(lutJ, lutH, lane);
+  smf_luts<true, 2, OT>(lutJ, lutH, lane);
   const ListRows R{rows, rows + WAVE, rows + 2 * WAVE};
   for (int64_t t = a.t0 + blockIdx.x; t < a.t1; t += gridDim.x) {
     const BatchTile tl = a.tiles[t];
@@ -133,10 +133,10 @@ batch_class_kernel(BatchArgs a, OT* __restrict__ F, OT* __restrict__ J, OT* __re
   constexpr int FAM = cls_fam(CLS), ORDER = cls_order(CLS);
   constexpr int B = FAM == 0 ? SmfLds<true, ORDER, OT>::BYTES : GmfLds<ORDER, OT>::BYTES;
   __shared__ __attribute__((aligned(16))) unsigned char lds[B];
-  __shared__ unsigned short lutJ[FAM == 0 && ORDER >= 1 ? 81 : 1], lutH[FAM == 0 && ORDER >= 2 ? 729 : 1];
+  __shared__ short lutJ[FAM == 0 && ORDER >= 1 ? 81 + LUT_PAD : 1], lutH[FAM == 0 && ORDER >= 2 ? 729 + LUT_PAD : 1];
   __shared__ int64_t rows[3 * WAVE];
   const int lane = threadIdx.x;
-  if (FAM == 0) smf_luts<true, ORDER>(lutJ, lutH, lane);
+  if (FAM == 0) smf_luts<true, ORDER, OT>(lutJ, lutH, lane);
   const ListRows R{rows, rows + WAVE, rows + 2 * WAVE};
   for (int64_t t = a.t0 + blockIdx.x; t < a.t1; t += gridDim.x) {
     const BatchTile tl = a.tiles[t];

@@ -27,10 +27,11 @@ __global__ void __launch_bounds__(WAVE) smf_kernel(int64_t n, const double* __re
   using L = SmfLds<MAT, ORDER, OT, HPP, SEP>;
   constexpr bool LUT = !PACK;
   __shared__ __attribute__((aligned(16))) unsigned char lds[L::BYTES];
-  __shared__ unsigned short lutJ[LUT && ORDER >= 1 ? smf_nj(MAT) : 1], lutH[LUT && ORDER >= 2 ? smf_nh(MAT) : 1];
+  __shared__ short lutJ[LUT && ORDER >= 1 ? smf_nj(MAT) + LUT_PAD : 1];
+  __shared__ short lutH[LUT && ORDER >= 2 ? smf_nh(MAT) + LUT_PAD : 1];
   const int64_t base = (int64_t)blockIdx.x * WAVE;
   const int nm = (int)(n - base < WAVE ? n - base : WAVE);
-  if (LUT) smf_luts<MAT, ORDER>(lutJ, lutH, threadIdx.x);
+  if (LUT) smf_luts<MAT, ORDER, OT>(lutJ, lutH, threadIdx.x);
   smf_tile<KIND, MAT, ORDER, OT, VEC, PA, RunRows, HPP, SEP, PACK>(RunRows{base}, nm, x, F, J, H, al, alv, c2.c, lds,
                                                                    lutJ, lutH, threadIdx.x);
 }

@@ -44,33 +44,35 @@ struct SmfLds {
 constexpr int smf_nj(bool mat) { return mat ? 81 : 36; }
 constexpr int smf_nh(bool mat) { return mat ? 729 : 216; }
 
-// lookup tables of the output layouts: distinct-value position | weight code << 8 (J: weight 2 if code 3; H: weight
-// 2^-code), filled by the wave
-template <bool MAT, int ORDER>
-MF_DEV void smf_luts(unsigned short* lutJ, unsigned short* lutH, int lane) {
+// lookup tables of the output layouts (the encoding of copy_out_lut), filled by the wave: J element e reads Js at
+// 6 + j_src of its matrix's phase-A tile (stride SA) with weight 2^1 where j_src's weight is 2; H element e reads
+// h_src of its matrix's H tile (stride SH) with weight 2^0, 2^-1 or 2^-2.  N + LUT_PAD entries: e >= N stands for
+// element e - N of the next matrix
+template <bool MAT, int ORDER, class OT>
+MF_DEV void smf_luts(short* lutJ, short* lutH, int lane) {
+  constexpr int NJ = smf_nj(MAT), NH = smf_nh(MAT);
   if (ORDER >= 1)
-    for (int e = lane; e < smf_nj(MAT); e += WAVE) {
+    for (int e = lane; e < NJ + LUT_PAD; e += WAVE) {
       double w;
-      const int s = j_src(MAT, e, w);
-      lutJ[e] = (unsigned short)(s | (w == 2.0 ? 3 << 8 : 0));
+      const int s = 6 + j_src(MAT, e % NJ, w) + (e >= NJ ? SA : 0);
+      lutJ[e] = lut_entry<OT>(s, w == 2.0 ? 1 : 0);
     }
   if (ORDER >= 2)
-    for (int e = lane; e < smf_nh(MAT); e += WAVE) {
+    for (int e = lane; e < NH + LUT_PAD; e += WAVE) {
       double w;
-      const int s = h_src(MAT, e, w);
-      lutH[e] = (unsigned short)(s | ((w == 1.0 ? 0 : (w == 0.5 ? 1 : 2)) << 8));
+      const int s = h_src(MAT, e % NH, w) + (e >= NH ? SH : 0);
+      lutH[e] = lut_entry<OT>(s, w == 1.0 ? 0 : (w == 0.5 ? -1 : -2));
     }
 }
 
 // one tile of nm <= 64 matrices; lds: SmfLds<MAT, ORDER, OT, HPP, SEP>::BYTES bytes, 16-byte aligned; the LUTs
 // filled (smf_luts) and visible to the wave; lane = threadIdx.x; VEC: elements per lane and store of J and H
-// (copy_out).  Ends with the tile's outputs written and a barrier (the
-// LDS may be reused).
+// (copy_out, copy_out_lut).  Ends with the tile's outputs written and a barrier (the LDS may be reused).
 template <int KIND, bool MAT, int ORDER, class OT, int VEC, bool PA, class R, int HPP = HPP_DEFAULT,
           bool SEP = false, bool PACK = false>
 MF_DEV void smf_tile(const R& rows, int nm, const double* __restrict__ x, OT* __restrict__ F, OT* __restrict__ J,
                      OT* __restrict__ H, double al, const double* __restrict__ alv, const double* __restrict__ c2,
-                     unsigned char* lds, const unsigned short* lutJ, const unsigned short* lutH, int lane) {
+                     unsigned char* lds, const short* lutJ, const short* lutH, int lane) {
   using L = SmfLds<MAT, ORDER, OT, HPP, SEP>;
   static_assert(!(PACK && MAT), "packed outputs take Voigt input");
   constexpr int NIN = L::NIN, NF = MAT ? 9 : 6, NJ = PACK ? 21 : NF * NF, NH = PACK ? 126 : NF * NF * NF;
@@ -129,13 +131,11 @@ MF_DEV void smf_tile(const R& rows, int nm, const double* __restrict__ x, OT* __
   }
   __syncthreads();
   copy_out<NF, 1, R_IN>(F, rows, NF * nm, lane, [&](int j, int e) { return t1[j * SA + f_src(MAT, e)]; });
-  if (ORDER >= 1)
-    copy_out<NJ, VEC, R_J>(J, rows, NJ * nm, lane, [&](int j, int e) {
-      if (PACK) return t1[j * SA + 6 + e];
-      const int u = lutJ[e];
-      const OT val = t1[j * SA + 6 + (u & 255)];
-      return (u >> 8) ? (OT)2 * val : val;
-    });
+  if (ORDER >= 1) {
+    if constexpr (PACK)
+      copy_out<NJ, VEC, R_J>(J, rows, NJ * nm, lane, [&](int j, int e) { return t1[j * SA + 6 + e]; });
+    else copy_out_lut<NJ, SA, VEC, R_J>(J, rows, NJ * nm, lane, t1, lutJ);
+  }
   if (ORDER >= 2) {
     if (!SEP) {
       __syncthreads();                         // (the phase-A tile fully read)
@@ -157,12 +157,9 @@ MF_DEV void smf_tile(const R& rows, int nm, const double* __restrict__ x, OT* __
       }
       __syncthreads();
       const R sub = shift_rows(rows, p0);
-      copy_out<NH, VEC, R_H>(H, sub, NH * cnt, lane, [&](int j, int e) {
-        if (PACK) return th[j * SH + e];
-        const int u = lutH[e];
-        const OT val = th[j * SH + (u & 255)];
-        return (u >> 8) == 0 ? val : ((u >> 8) == 1 ? (OT)0.5 * val : (OT)0.25 * val);
-      });
+      if constexpr (PACK)
+        copy_out<NH, VEC, R_H>(H, sub, NH * cnt, lane, [&](int j, int e) { return th[j * SH + e]; });
+      else copy_out_lut<NH, SH, VEC, R_H>(H, sub, NH * cnt, lane, th, lutH);
     }
   }
   __syncthreads();                             // (the LDS is free for the next tile)

@@ -114,6 +114,62 @@ MF_DEV void copy_out(OT* __restrict__ out, const R& rows, int count, int lane, G
   }
 }
 
+// Output layouts through a lookup table: element e of a matrix's output row is a value staged in LDS (the matrix's
+// block: S elements per matrix) times a power of two.  Entry lut[e] (a short): bits 0..11 the byte offset of that
+// value from the matrix's block, bits 12..15 the exponent x (signed: -8 .. 7); out = value 2^x.  Entries N .. N +
+// LUT_PAD - 1 describe elements 0 .. LUT_PAD - 1 of the next matrix (offset + S elements), for vector stores that
+// run across a matrix boundary.
+constexpr int LUT_PAD = 3;                                   // (vectors of up to 4 elements)
+template <class OT>
+MF_DEV short lut_entry(int src, int x) {                     // src: staged element (< 4096 / sizeof(OT))
+  return (short)(unsigned short)((src * (int)sizeof(OT)) | ((x & 15) << 12));
+}
+template <class OT>
+MF_DEV OT lut_value(const unsigned char* blk, int u) {       // (ldexp: the exact product by 2^x)
+  const OT v = *reinterpret_cast<const OT*>(blk + (u & 0xfff));
+  if constexpr (sizeof(OT) == 4) return __builtin_ldexpf(v, u >> 12);
+  else return __builtin_ldexp(v, u >> 12);
+}
+
+// copy_out of a layout with a lookup table: out[row(j) N + e] = lut_value(staged block of matrix j, lut[e]), blocks
+// at st + j S.  FP32 outputs, RunRows, VEC > 1: each lane keeps the offset e of its vector's first element and a
+// pointer to that element's block; the table's entries past N cover the elements that fall into the next matrix, so
+// the per-element work is two LDS reads, the address sum and the scaling (the mixed kernels are limited by this
+// VALU work).  FP64 outputs take copy_out's per-element form: with the FP64 kernels' six wavefronts per CU and the
+// store rate their bound, it measured 2-3% faster on MI355X than the short loop (same results either way).
+template <int N, int S, int VEC, int REG, class R, class OT>
+MF_DEV void copy_out_lut(OT* __restrict__ out, const R& rows, int count, int lane, const OT* st, const short* lut) {
+  static_assert(VEC - 1 <= LUT_PAD, "vector longer than the table's padding");
+  const unsigned char* st8 = reinterpret_cast<const unsigned char*>(st);
+  if constexpr (R::RUN && VEC > 1 && sizeof(OT) == 4) {
+    typedef OT vv __attribute__((ext_vector_type(VEC)));
+    OT* blk = out + row_of<REG>(rows, 0) * N;
+    int t = VEC * lane, j = t / N, e = t - j * N;
+    const unsigned char* b = st8 + j * S * (int)sizeof(OT);
+    for (; t + VEC - 1 < count; t += VEC * WAVE) {
+      vv val;
+#pragma unroll
+      for (int q = 0; q < VEC; q++) val[q] = lut_value<OT>(b, lut[e + q]);
+      *reinterpret_cast<vv*>(blk + t) = val;
+      e += VEC * WAVE;
+      if constexpr (N >= VEC * WAVE) {           // (at most one matrix boundary per step: selects, no division)
+        const bool w = e >= N;
+        e = w ? e - N : e;
+        b = w ? b + S * (int)sizeof(OT) : b;
+      } else {
+        while (e >= N) { e -= N; b += S * (int)sizeof(OT); }
+      }
+    }
+    for (; t < count; t++) {                     // (the tail: at most one lane, fewer than VEC elements)
+      blk[t] = lut_value<OT>(b, lut[e]);
+      if (++e == N) { e = 0; b += S * (int)sizeof(OT); }
+    }
+  } else {
+    copy_out<N, VEC, REG>(out, rows, count, lane,
+                          [&](int j, int e) { return lut_value<OT>(st8 + j * S * (int)sizeof(OT), lut[e]); });
+  }
+}
+
 // the tile's inputs (NIN doubles per matrix) into LDS, coalesced
 template <int NIN, class R>
 MF_DEV void load_in(double* __restrict__ xin, const double* __restrict__ x, const R& rows, int nm, int lane) {
