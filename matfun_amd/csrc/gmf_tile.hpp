@@ -20,17 +20,17 @@ namespace mf {
 static __constant__ GmfRecip g_gmf_recip = gmf_recip_table();
 
 // LDS layout (elements of OT):
-//   main [64][MS]   r1 (9), r2 (9); order 2: the 10 distinct D3 and sc2 (MS = 29; order 1: 19)
+//   main [64][MS]   r1 (9), r2 (9); order 2: the 10 distinct D3 (MS = 28; order 1: 19)
 //   x    [XS]       phase 1: F of the 64 matrices [0, 576), then D2 (6 distinct) and sc1 per matrix [XAUX, +448);
-//                   J passes: the JMPP x 81 output tile [0, XAUX); H passes: a half-block tile 2 x 378 (or 2 x 351)
+//                   J passes: the JMPP x 81 output tile [0, XAUX); H passes: the 729 values of one matrix
 // (the FP64 input block of 64 x 9 doubles overlays the start, before phase 1)
 template <int ORDER, class OT>
 struct GmfLds {
-  static constexpr int MS = ORDER >= 2 ? 29 : 19;
+  static constexpr int MS = ORDER >= 2 ? 28 : 19;
   static constexpr int JG = ORDER == 1 ? 9 : 27;            // J: lanes per matrix
   static constexpr int JMPP = WAVE / JG;                    // J: matrices per pass
   static constexpr int XAUX = JMPP * 81;
-  static constexpr int XS = ORDER == 0 ? 576 : (XAUX + 448 > 756 ? XAUX + 448 : 756);
+  static constexpr int XS = ORDER == 0 ? 576 : (ORDER == 1 || XAUX + 448 > 729 ? XAUX + 448 : 729);
   static constexpr int MAIN = ORDER == 0 ? 0 : 64 * MS;
   static constexpr int TOT = MAIN + XS;
   static constexpr int BYTES = (TOT * (int)sizeof(OT) > WAVE * 9 * 8) ? TOT * (int)sizeof(OT) : WAVE * 9 * 8;
@@ -43,6 +43,7 @@ MF_DEV void gmf_tile(const R& rows, int nm, const double* __restrict__ A, OT* __
                      OT* __restrict__ H, double al, const double* __restrict__ alv, const double* __restrict__ bt,
                      unsigned char* lds, int lane) {
   using L = GmfLds<ORDER, OT>;
+  constexpr int VEC = 16 / (int)sizeof(OT);             // (elements per 16-byte store)
   double* xin = reinterpret_cast<double*>(lds);
   OT* mainr = reinterpret_cast<OT*>(lds);
   OT* xr = mainr + L::MAIN;
@@ -70,6 +71,7 @@ MF_DEV void gmf_tile(const R& rows, int nm, const double* __restrict__ A, OT* __
     o.sc1 *= nanf;
     o.sc2 *= nanf;
   }
+  const OT sc2 = ORDER >= 2 ? (OT)o.sc2 : (OT)0;     // (order 2: H's factor, read by the H lanes through a shuffle)
   if (ORDER >= 1) {
     OT* my = mainr + lane * L::MS;
 #pragma unroll
@@ -81,13 +83,13 @@ MF_DEV void gmf_tile(const R& rows, int nm, const double* __restrict__ A, OT* __
         for (int j = i; j < 3; j++)
 #pragma unroll
           for (int k = j; k < 3; k++) my[18 + sym33s(i, j, k)] = (OT)o.D3[9 * i + 3 * j + k];
-      my[28] = (OT)o.sc2;
     }
   }
 #pragma unroll
   for (int k = 0; k < 9; k++) xr[lane * 9 + k] = (OT)o.F[k];
   __syncthreads();
-  copy_out<9, 1, R_IN>(F, rows, 9 * nm, lane, [&](int j, int e) { return xr[9 * j + e]; });
+  if constexpr (R::RUN) copy_block<VEC>(F + rows.m(0) * 9, xr, 9 * nm, lane);
+  else copy_out<9, 1, R_IN>(F, rows, 9 * nm, lane, [&](int j, int e) { return xr[9 * j + e]; });
   if (ORDER >= 1) {
     __syncthreads();
 #pragma unroll
@@ -122,7 +124,8 @@ MF_DEV void gmf_tile(const R& rows, int nm, const double* __restrict__ A, OT* __
         }
       }
       __syncthreads();
-      copy_out<81, 1, R_J>(J, shift_rows(rows, p0), 81 * cnt, lane, [&](int j, int e) { return xr[81 * j + e]; });
+      if constexpr (R::RUN) copy_block<VEC>(J + rows.jr(p0) * 81, xr, 81 * cnt, lane);
+      else copy_out<81, 1, R_J>(J, shift_rows(rows, p0), 81 * cnt, lane, [&](int j, int e) { return xr[81 * j + e]; });
       __syncthreads();
     }
     if (ORDER >= 2) {
@@ -131,30 +134,18 @@ MF_DEV void gmf_tile(const R& rows, int nm, const double* __restrict__ A, OT* __
       for (int p0 = 0; p0 < nm; p0 += MPP) {        // H: H_abcdef, (a,b,c) = 0..26 at 27 (9a + 3b + c) + (9d + 3e + f)
         const int cnt = nm - p0 < MPP ? nm - p0 : MPP;
         OT out[27];
+        const OT s2 = __shfl(sc2, (p0 + g) & (WAVE - 1));   // (sc2 of the lane group's matrix, from its lane)
         if (g < cnt) {
           const OT* s = mainr + (p0 + g) * L::MS;
-          gmf_h_lane(s, s + 9, s + 18, s[28], Lg / 9, (Lg / 3) % 3, Lg % 3, out);
+          gmf_h_lane(s, s + 9, s + 18, s2, Lg / 9, (Lg / 3) % 3, Lg % 3, out);
         }
-        const R sub = shift_rows(rows, p0);
+        for (int gg = 0; gg < cnt; gg++) {         // one matrix at a time: its 729 values staged, then stored
+          if (g == gg) {
 #pragma unroll
-        for (int half = 0; half < 2; half++) {      // rows k < 14 (378 elements), then k >= 14 (351)
-          constexpr int K0[2] = {0, 14}, SZ[2] = {378, 351};
-          if (g < cnt) {
-#pragma unroll
-            for (int k = 0; k < 14; k++)
-              if (K0[half] + k < 27) xr[g * SZ[half] + 27 * k + Lg] = out[K0[half] + k];
+            for (int k = 0; k < 27; k++) xr[27 * k + Lg] = out[k];
           }
           __syncthreads();
-          if (R::RUN) {
-            for (int t = lane; t < SZ[half] * cnt; t += WAVE) {
-              const int gg = t >= SZ[half] ? 1 : 0, e = t - gg * SZ[half];
-              H[(sub.hr(0) + gg) * 729 + 27 * K0[half] + e] = xr[t];
-            }
-          } else {
-            OT* h0 = H + sub.hr(0) * 729 + 27 * K0[half];
-            OT* h1 = cnt > 1 ? H + sub.hr(1) * 729 + 27 * K0[half] : h0;
-            for (int t = lane; t < SZ[half] * cnt; t += WAVE) *(t >= SZ[half] ? h1 + (t - SZ[half]) : h0 + t) = xr[t];
-          }
+          copy_block<VEC>(H + rows.hr(p0 + gg) * 729, xr, 729, lane);
           __syncthreads();
         }
       }

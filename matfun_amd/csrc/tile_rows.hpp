@@ -170,6 +170,30 @@ MF_DEV void copy_out_lut(OT* __restrict__ out, const R& rows, int count, int lan
   }
 }
 
+// out[0 .. count) = src[0 .. count) (src in LDS), by the wave, in stores of VEC elements aligned to VEC sizeof(OT)
+// bytes: the head (the elements before the first aligned vector) and the tail one element per lane, the rest in
+// vectors; out needs only the alignment of OT
+template <int VEC, class OT>
+MF_DEV void copy_block(OT* __restrict__ out, const OT* src, int count, int lane) {
+  const int mis = (int)(((uintptr_t)out / sizeof(OT)) % VEC);
+  const int head = (VEC - mis) % VEC < count ? (VEC - mis) % VEC : count;
+  if (lane < head) out[lane] = src[lane];
+  const int nv = (count - head) / VEC;
+  if constexpr (VEC > 1) {
+    typedef OT vv __attribute__((ext_vector_type(VEC)));
+    for (int v = lane; v < nv; v += WAVE) {
+      vv val;
+#pragma unroll
+      for (int q = 0; q < VEC; q++) val[q] = src[head + VEC * v + q];
+      *reinterpret_cast<vv*>(out + head + VEC * v) = val;
+    }
+  } else {
+    for (int v = lane; v < nv; v += WAVE) out[head + v] = src[head + v];
+  }
+  const int t0 = head + VEC * nv;
+  if (lane < count - t0) out[t0 + lane] = src[t0 + lane];
+}
+
 // the tile's inputs (NIN doubles per matrix) into LDS, coalesced
 template <int NIN, class R>
 MF_DEV void load_in(double* __restrict__ xin, const double* __restrict__ x, const R& rows, int nm, int lane) {
