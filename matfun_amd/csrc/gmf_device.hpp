@@ -202,6 +202,71 @@ MF_DEV int series_terms(const double* __restrict__ b, double al, int n, double r
   return n + M + 1 < GMF_KMAX ? n + M + 1 : GMF_KMAX;
 }
 
+// the group series of one matrix (gmf_kernel.hpp:478-533): K terms (0: no group) of sum_j b_j E_j with the
+// offsets u of the nodes from the group centre, the pole factor 1 / ap (iap) of a node outside a pair group, the
+// first coefficient b0 (log: its value at the centre), the exponent al (per-matrix exponents), and the group and
+// pole flags (bits 0..2: node in the group; 3..5: the pole node)
+struct GmfSeriesIn {
+  cpx u[3];
+  double iap, b0, al;
+  int K, flags;
+};
+
+// acc[c] = the series sums of the ND = 3 / 9 / 19 entries of the divided-difference list (order 0 / 1 / 2), streamed
+// over the series index j; bt: the coefficients b_j (PA: 1 / j, the coefficients formed per lane)
+template <int KIND, int ORDER, bool PA>
+MF_DEV void gmf_series(const GmfSeriesIn& s, const double* __restrict__ bt, cpx* acc) {
+  constexpr int ND = ORDER == 0 ? 3 : (ORDER == 1 ? 9 : 19);
+  const bool inG[3] = {(s.flags & 1) != 0, (s.flags & 2) != 0, (s.flags & 4) != 0};
+  const bool pk[3] = {(s.flags & 8) != 0, (s.flags & 16) != 0, (s.flags & 32) != 0};
+  const double iap = s.iap, al = s.al;
+  cpx E[ND];
+#pragma unroll
+  for (int c = 0; c < ND; c++) {
+    const int par = dd_parent(c);
+    E[c] = inG[dd_added(c)] ? mk(0.0, 0.0) : (par < 0 ? mk(1.0, 0.0) : E[par]);
+    acc[c] = mk(0.0, 0.0);
+  }
+  double q1 = 0.0, q2 = 0.0, q3 = 0.0;
+  double bnext = 0.0;                           // (b_j loaded one iteration ahead of its use)
+  for (int j = 0; j < s.K; j++) {
+    const double bj = j == 0 ? s.b0 : bnext;
+    bnext = PA ? bj * ((al - j) * bt[j + 1 < GMF_KMAX ? j + 1 : GMF_KMAX - 1])
+               : bt[j + 1 < GMF_KMAX ? j + 1 : GMF_KMAX - 1];
+    const double g1 = (bj - q1) * iap, g2 = (g1 - q2) * iap, g3 = (g2 - q3) * iap;
+    q1 = g1; q2 = g2; q3 = g3;
+    // E in place: children after their parents in the list, so a descending pass sees the parents' values at
+    // j - 1 (a child whose node is in the group: parent(j-1) + u E(j-1)), and an ascending pass then copies the
+    // parent's new value into a child whose node is outside the group.  (The descending pass updates every entry:
+    // an entry outside the group is read by its children before it is updated, and overwritten by the copy.)
+#pragma unroll
+    for (int c = ND - 1; c >= 0; c--) {
+      const int par = dd_parent(c), i = dd_added(c);
+      const cpx epp = par < 0 ? mk(j == 0 ? 1.0 : 0.0, 0.0) : E[par];
+      E[c] = add(epp, mul(s.u[i], E[c]));
+    }
+#pragma unroll
+    for (int c = 0; c < ND; c++) {
+      const int par = dd_parent(c), i = dd_added(c);
+      E[c] = pick(inG[i], E[c], par < 0 ? mk(0.0, 0.0) : E[par]);
+      // pole factor: the multiplicity of the node outside a pair group selects G^(r) (masks loop-invariant,
+      // indices constant)
+      const double qa = dd_a(c) == 0 ? bj : (dd_a(c) == 1 ? g1 : (dd_a(c) == 2 ? g2 : g3));
+      const double qb = dd_b(c) == 0 ? bj : (dd_b(c) == 1 ? g1 : (dd_b(c) == 2 ? g2 : g3));
+      const double qc = dd_c(c) == 0 ? bj : (dd_c(c) == 1 ? g1 : (dd_c(c) == 2 ? g2 : g3));
+      const double g = pk[0] ? qa : (pk[1] ? qb : (pk[2] ? qc : bj));
+      acc[c] = add(acc[c], scale(E[c], g));
+    }
+  }
+}
+
+// the series computed by the lane itself
+template <int KIND, int ORDER, bool PA>
+struct GmfSeriesLocal {
+  const double* bt;
+  MF_DEV void operator()(const GmfSeriesIn& s, cpx* acc) const { gmf_series<KIND, ORDER, PA>(s, bt, acc); }
+};
+
 // results of one matrix needed for the output assembly
 struct GmfOut {
   double F[9];
@@ -212,10 +277,11 @@ struct GmfOut {
 };
 
 // PA (powers with a per-matrix exponent al): bt holds 1 / j instead of the coefficients b_j (gmf_recips)
-// rinv: gmf_recip_table()
-template <int KIND, int ORDER, bool PA = false>
+// rinv: gmf_recip_table(); series(in, acc): the group series (GmfSeriesLocal, or one shared by the lanes of a
+// workgroup), called by every lane (K = 0 without a group)
+template <int KIND, int ORDER, bool PA, class SER>
 MF_DEV void gmf_lane(const double* a, double al, const double* __restrict__ bt, GmfOut& o,
-                     const double* __restrict__ rinv) {
+                     const double* __restrict__ rinv, const SER& series) {
   constexpr int ND = ORDER == 0 ? 3 : (ORDER == 1 ? 9 : 19);
   const double RHO = 0.7, R_EXP = 4.0, RHO_POLE = 0.6;
   double amax = fabs(a[0]);
@@ -445,60 +511,33 @@ MF_DEV void gmf_lane(const double* a, double al, const double* __restrict__ bt, 
   cpx D[ND];
 #pragma unroll
   for (int c = 0; c < ND; c++) D[c] = mk(0.0, 0.0);
-  if (hasg) {
-    const int K = series_terms<PA>(bt, al, 3 * (ORDER + 1) - 1, rr, rq, rinv);
-    const double b0 = KIND == LOG ? node_value<LOG>(al, sig, m, mk(cxg, 0.0), mk(cmu, 0.0), 0).re : 1.0;
-    cpx E[ND], acc[ND];
+  {
+    GmfSeriesIn sin;
+    sin.K = hasg ? series_terms<PA>(bt, al, 3 * (ORDER + 1) - 1, rr, rq, rinv) : 0;
+    sin.b0 = (hasg && KIND == LOG) ? node_value<LOG>(al, sig, m, mk(cxg, 0.0), mk(cmu, 0.0), 0).re : 1.0;
 #pragma unroll
-    for (int c = 0; c < ND; c++) {
-      const int par = dd_parent(c);
-      E[c] = inG[dd_added(c)] ? mk(0.0, 0.0) : (par < 0 ? mk(1.0, 0.0) : E[par]);
-      acc[c] = mk(0.0, 0.0);
-    }
-    double q1 = 0.0, q2 = 0.0, q3 = 0.0;
-    double bnext = 0.0;                         // (b_j loaded one iteration ahead of its use)
-    for (int j = 0; j < K; j++) {
-      const double bj = j == 0 ? b0 : bnext;
-      bnext = PA ? bj * ((al - j) * bt[j + 1 < GMF_KMAX ? j + 1 : GMF_KMAX - 1])
-                 : bt[j + 1 < GMF_KMAX ? j + 1 : GMF_KMAX - 1];
-      const double g1 = (bj - q1) * iap, g2 = (g1 - q2) * iap, g3 = (g2 - q3) * iap;
-      q1 = g1; q2 = g2; q3 = g3;
-      // E in place: children after their parents in the list, so a descending pass sees the parents' values at
-      // j - 1 (a child whose node is in the group: parent(j-1) + u E(j-1)), and an ascending pass then copies the
-      // parent's new value into a child whose node is outside the group.  (The descending pass updates every entry:
-      // an entry outside the group is read by its children before it is updated, and overwritten by the copy.)
-#pragma unroll
-      for (int c = ND - 1; c >= 0; c--) {
-        const int par = dd_parent(c), i = dd_added(c);
-        const cpx epp = par < 0 ? mk(j == 0 ? 1.0 : 0.0, 0.0) : E[par];
-        E[c] = add(epp, mul(u[i], E[c]));
-      }
+    for (int i = 0; i < 3; i++) sin.u[i] = u[i];
+    sin.iap = iap;
+    sin.al = al;
+    sin.flags = (inG[0] ? 1 : 0) | (inG[1] ? 2 : 0) | (inG[2] ? 4 : 0) | (pk[0] ? 8 : 0) | (pk[1] ? 16 : 0) |
+                (pk[2] ? 32 : 0);
+    cpx acc[ND];
+    series(sin, acc);
+    if (hasg) {
 #pragma unroll
       for (int c = 0; c < ND; c++) {
-        const int par = dd_parent(c), i = dd_added(c);
-        E[c] = pick(inG[i], E[c], par < 0 ? mk(0.0, 0.0) : E[par]);
-        // pole factor: the multiplicity of the node outside a pair group selects G^(r) (masks loop-invariant,
-        // indices constant)
-        const double qa = dd_a(c) == 0 ? bj : (dd_a(c) == 1 ? g1 : (dd_a(c) == 2 ? g2 : g3));
-        const double qb = dd_b(c) == 0 ? bj : (dd_b(c) == 1 ? g1 : (dd_b(c) == 2 ? g2 : g3));
-        const double qc = dd_c(c) == 0 ? bj : (dd_c(c) == 1 ? g1 : (dd_c(c) == 2 ? g2 : g3));
-        const double g = pk[0] ? qa : (pk[1] ? qb : (pk[2] ? qc : bj));
-        acc[c] = add(acc[c], scale(E[c], g));
-      }
-    }
+        const int n = dd_a(c) + dd_b(c) + dd_c(c) - 1, k = dd_a(c) - 1;
+        double fac;
+        if (KIND == EXP) {
+          fac = exp(cxg - xshift);
+        } else {
+          double iLn = 1.0;
 #pragma unroll
-    for (int c = 0; c < ND; c++) {
-      const int n = dd_a(c) + dd_b(c) + dd_c(c) - 1, k = dd_a(c) - 1;
-      double fac;
-      if (KIND == EXP) {
-        fac = exp(cxg - xshift);
-      } else {
-        double iLn = 1.0;
-#pragma unroll
-        for (int r = 0; r < n - (KIND == POW ? k : 0); r++) iLn = iLn * iL;
-        fac = KIND == LOG ? iLn : powk(sig * L, al, (double)k) * iLn;
+          for (int r = 0; r < n - (KIND == POW ? k : 0); r++) iLn = iLn * iL;
+          fac = KIND == LOG ? iLn : powk(sig * L, al, (double)k) * iLn;
+        }
+        D[c] = scale(acc[c], fac);
       }
-      D[c] = scale(acc[c], fac);
     }
   }
   // ---- residues of the single nodes (gmf_kernel.hpp:534-576)

@@ -2,10 +2,10 @@
 //   phase 1, one matrix per lane: eigenvalues, divided differences, real basis, contracted tensors (gmf_device.hpp);
 //            r_m (27 values) and D3 (27) staged in LDS per matrix, D2 (9) and F (9) in a second region
 //   phase 2, lane groups, assembling J and H into an LDS tile in the API layout that the wave then writes as one
-//            contiguous block: J with 9 lanes per matrix (9 values each), 7 matrices per pass (order 1) or 27 lanes
-//            per matrix (3 values each), 2 matrices per pass (order 2, where the smaller J tile keeps the LDS of a
-//            wavefront at 20.9 KB); H with 27 lanes per matrix (27 values each), 2 matrices per pass (2 x 5.8 KB of
-//            consecutive output per pass)
+//            contiguous block (16-byte stores for consecutive rows, copy_block): J with 9 lanes per matrix (9 values
+//            each), 7 matrices per pass (order 1) or 27 lanes per matrix (3 values each), 2 matrices per pass (order
+//            2, where the smaller J tile keeps the LDS of a wavefront under 20 KB: eight per CU in FP64); H with 27
+//            lanes per matrix (27 values each), 2 matrices per pass, stored one matrix (5.8 KB FP64) at a time
 // OT = double: the FP64 form; OT = float: the mixed form (FP64 per-lane math, FP32 staged tensors, J / H assembly
 // and outputs).  PA: powers with one exponent per matrix (alv[i], read by its lane; bt then holds 1 / j; a matrix
 // whose exponent is not finite gets NaN results).  Used by the homogeneous kernels (gmf_kernels.hip, RunRows) and the
@@ -18,6 +18,59 @@ namespace mf {
 
 // the reciprocals of the series-length bound, in constant memory (uniform reads: scalar loads)
 static __constant__ GmfRecip g_gmf_recip = gmf_recip_table();
+
+// The group series of the NW x 64 matrices of a workgroup (NW tiles), dealt out by length: a wavefront runs to the
+// longest series among its lanes, and the lengths spread widely (log, order 1: median 37 terms, 1% above 147), so the
+// lanes post their series (inputs and term count) to LDS, rank them by term count, and lane t of the workgroup
+// computes the series of rank t; the sums return through LDS to the lanes that posted them.  Same arithmetic as the
+// lane's own series (gmf_series): the results do not depend on where a series runs.
+// LDS pool: GmfPool::BYTES, shared by the workgroup's tiles and free while they are in phase 1
+template <int ORDER, int NW>
+struct GmfPool {
+  static constexpr int NJ = WAVE * NW, ND = ORDER == 0 ? 3 : (ORDER == 1 ? 9 : 19);
+  static constexpr int IN = NJ * ((int)sizeof(GmfSeriesIn) + 4 + 2);      // inputs, term counts, job of each rank
+  static constexpr int OUT = NJ * ND * (int)sizeof(cpx);                  // sums, [entry][job]
+  static constexpr int BYTES = IN > OUT ? IN : OUT;
+};
+template <int KIND, int ORDER, bool PA, int NW>
+struct GmfSeriesShared {
+  const double* bt;
+  unsigned char* pool;            // LDS, 16-byte aligned
+  int tid;                        // threadIdx.x
+  // every lane of the workgroup calls it once per tile (five workgroup barriers)
+  __device__ void operator()(const GmfSeriesIn& s, cpx* acc) const {
+    using P = GmfPool<ORDER, NW>;
+    GmfSeriesIn* jin = reinterpret_cast<GmfSeriesIn*>(pool);
+    int* key = reinterpret_cast<int*>(pool + P::NJ * sizeof(GmfSeriesIn));
+    short* job_of = reinterpret_cast<short*>(key + P::NJ);
+    __syncthreads();                                   // (every tile has read its input block: the pool is free)
+    jin[tid] = s;
+    key[tid] = s.K;
+    __syncthreads();
+    int r = 0;                                         // rank: longer series first, ties by position
+    for (int t = 0; t < P::NJ; t += 4) {
+      const int4 k4 = *reinterpret_cast<const int4*>(key + t);
+      r += (k4.x > s.K || (k4.x == s.K && t < tid)) ? 1 : 0;
+      r += (k4.y > s.K || (k4.y == s.K && t + 1 < tid)) ? 1 : 0;
+      r += (k4.z > s.K || (k4.z == s.K && t + 2 < tid)) ? 1 : 0;
+      r += (k4.w > s.K || (k4.w == s.K && t + 3 < tid)) ? 1 : 0;
+    }
+    job_of[r] = (short)tid;
+    __syncthreads();
+    const int job = job_of[tid];
+    const GmfSeriesIn mine = jin[job];
+    __syncthreads();                                   // (inputs read: the pool takes the sums)
+    cpx a[P::ND];
+    gmf_series<KIND, ORDER, PA>(mine, bt, a);
+    cpx* res = reinterpret_cast<cpx*>(pool);
+#pragma unroll
+    for (int c = 0; c < P::ND; c++) res[c * P::NJ + job] = a[c];
+    __syncthreads();
+#pragma unroll
+    for (int c = 0; c < P::ND; c++) acc[c] = res[c * P::NJ + tid];
+    __syncthreads();                                   // (sums read: the tiles may stage into the pool)
+  }
+};
 
 // LDS layout (elements of OT):
 //   main [64][MS]   r1 (9), r2 (9); order 2: the 10 distinct D3 (MS = 28; order 1: 19)
@@ -36,12 +89,15 @@ struct GmfLds {
   static constexpr int BYTES = (TOT * (int)sizeof(OT) > WAVE * 9 * 8) ? TOT * (int)sizeof(OT) : WAVE * 9 * 8;
 };
 
-// one tile of nm <= 64 matrices; lds: GmfLds<ORDER, OT>::BYTES bytes, 16-byte aligned; lane = threadIdx.x.  Ends
-// with the tile's outputs written and a barrier (the LDS may be reused).
-template <int KIND, int ORDER, class OT, bool PA, class R>
+// one tile of nm <= 64 matrices (nm = 0: none, the wave still takes part in the shared series); lds:
+// GmfLds<ORDER, OT>::BYTES bytes, 16-byte aligned; lane: the lane in the tile's wavefront.  NW = 1: each lane computes
+// its own series; NW > 1: the tile is one of NW in a workgroup of NW wavefronts (every one calling gmf_tile once) that
+// share out the series (GmfSeriesShared; pool: its LDS, tid = threadIdx.x).  Ends with the tile's outputs written and
+// a wave barrier (the LDS may be reused).
+template <int KIND, int ORDER, class OT, bool PA, class R, int NW = 1>
 MF_DEV void gmf_tile(const R& rows, int nm, const double* __restrict__ A, OT* __restrict__ F, OT* __restrict__ J,
                      OT* __restrict__ H, double al, const double* __restrict__ alv, const double* __restrict__ bt,
-                     unsigned char* lds, int lane) {
+                     unsigned char* lds, int lane, unsigned char* pool = nullptr, int tid = 0) {
   using L = GmfLds<ORDER, OT>;
   constexpr int VEC = 16 / (int)sizeof(OT);             // (elements per 16-byte store)
   double* xin = reinterpret_cast<double*>(lds);
@@ -49,7 +105,7 @@ MF_DEV void gmf_tile(const R& rows, int nm, const double* __restrict__ A, OT* __
   OT* xr = mainr + L::MAIN;
 
   load_in<9>(xin, A, rows, nm, lane);
-  __syncthreads();
+  wave_sync();
   double a[9];
 #pragma unroll
   for (int k = 0; k < 9; k++) a[k] = lane < nm ? xin[9 * lane + k] : (k % 4 == 0 ? 1.0 : 0.0);
@@ -59,11 +115,14 @@ MF_DEV void gmf_tile(const R& rows, int nm, const double* __restrict__ A, OT* __
     alok = fabs(al) <= 1.7976931348623157e308;
     al = alok ? al : 1.0;
   }
-  __syncthreads();
+  wave_sync();
 
   // ---- phase 1
   GmfOut o;
-  gmf_lane<KIND, ORDER, PA>(a, al, bt, o, g_gmf_recip.r);
+  if constexpr (NW > 1)
+    gmf_lane<KIND, ORDER, PA>(a, al, bt, o, g_gmf_recip.r, GmfSeriesShared<KIND, ORDER, PA, NW>{bt, pool, tid});
+  else
+    gmf_lane<KIND, ORDER, PA>(a, al, bt, o, g_gmf_recip.r, GmfSeriesLocal<KIND, ORDER, PA>{bt});
   if (PA) {                                          // (J and H carry the factors sc1, sc2)
     const double nanf = alok ? 1.0 : dnan();
 #pragma unroll
@@ -87,17 +146,17 @@ MF_DEV void gmf_tile(const R& rows, int nm, const double* __restrict__ A, OT* __
   }
 #pragma unroll
   for (int k = 0; k < 9; k++) xr[lane * 9 + k] = (OT)o.F[k];
-  __syncthreads();
+  wave_sync();
   if constexpr (R::RUN) copy_block<VEC>(F + rows.m(0) * 9, xr, 9 * nm, lane);
   else copy_out<9, 1, R_IN>(F, rows, 9 * nm, lane, [&](int j, int e) { return xr[9 * j + e]; });
   if (ORDER >= 1) {
-    __syncthreads();
+    wave_sync();
 #pragma unroll
     for (int i = 0; i < 3; i++)
 #pragma unroll
       for (int j = i; j < 3; j++) xr[L::XAUX + lane * 7 + sym3(i, j)] = (OT)o.D2[3 * i + j];
     xr[L::XAUX + lane * 7 + 6] = (OT)o.sc1;
-    __syncthreads();
+    wave_sync();
 
     // ---- phase 2, J: lane (gj, Lj) = matrix gj of the pass, Lj = 9 b + 3 c + d (27 lanes: J_abcd for a = 0..2) or
     //      Lj = 3 c + d (9 lanes: J_abcd for a, b = 0..2), each value at 27 a + 9 b + 3 c + d of its matrix
@@ -123,10 +182,10 @@ MF_DEV void gmf_tile(const R& rows, int nm, const double* __restrict__ A, OT* __
           }
         }
       }
-      __syncthreads();
+      wave_sync();
       if constexpr (R::RUN) copy_block<VEC>(J + rows.jr(p0) * 81, xr, 81 * cnt, lane);
       else copy_out<81, 1, R_J>(J, shift_rows(rows, p0), 81 * cnt, lane, [&](int j, int e) { return xr[81 * j + e]; });
-      __syncthreads();
+      wave_sync();
     }
     if (ORDER >= 2) {
       constexpr int G = 27, MPP = WAVE / G;            // H: lane (g, Lg), Lg = 9 d + 3 e + f
@@ -144,14 +203,14 @@ MF_DEV void gmf_tile(const R& rows, int nm, const double* __restrict__ A, OT* __
 #pragma unroll
             for (int k = 0; k < 27; k++) xr[27 * k + Lg] = out[k];
           }
-          __syncthreads();
+          wave_sync();
           copy_block<VEC>(H + rows.hr(p0 + gg) * 729, xr, 729, lane);
-          __syncthreads();
+          wave_sync();
         }
       }
     }
   }
-  __syncthreads();                                   // (the LDS is free for the next tile)
+  wave_sync();                                   // (the LDS is free for the next tile)
 }
 
 }  // namespace mf

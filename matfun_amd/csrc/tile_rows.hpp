@@ -29,6 +29,15 @@ struct ListRows {
   MF_DEV int64_t hr(int j) const { return hrr[j]; }
 };
 
+// the barrier of one tile's wavefront: the ordering of __syncthreads (LDS writes before it visible to the wave's
+// reads after it) without the workgroup's s_barrier, so that the tiles of a workgroup of several wavefronts run their
+// phases independently (for a workgroup of one wavefront __syncthreads compiles to the same)
+__device__ inline void wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+}
+
 // a copy of v / p that the compiler cannot see through (an empty asm statement): what a tile computes from it is not
 // hoisted out of a loop over tiles, so a persistent kernel keeps no per-tile state live from one tile to the next
 MF_DEV int opaque(int v) {

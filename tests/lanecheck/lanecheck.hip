@@ -66,7 +66,7 @@ void gmf_all(double al, const double* alv, int64_t n, const double* A, double* F
     if (PA) al = exponent_of(alv, m, nanf);
     mf::GmfOut o;
     static constexpr mf::GmfRecip rinv = mf::gmf_recip_table();
-    mf::gmf_lane<KIND, ORDER, PA>(A + 9 * m, al, bt.b, o, rinv.r);
+    mf::gmf_lane<KIND, ORDER, PA>(A + 9 * m, al, bt.b, o, rinv.r, mf::GmfSeriesLocal<KIND, ORDER, PA>{bt.b});
     for (int k = 0; k < 9; k++) o.F[k] *= nanf;
     o.sc1 *= nanf;
     o.sc2 *= nanf;
