@@ -173,6 +173,26 @@ MF_DEV void copy_out_lut(OT* __restrict__ out, const R& rows, int count, int lan
       blk[t] = lut_value<OT>(b, lut[e]);
       if (++e == N) { e = 0; b += S * (int)sizeof(OT); }
     }
+  } else if constexpr (!R::RUN && N >= 4 * WAVE) {
+    // rows listed, long rows (H): each matrix's row by itself, in 16-byte stores after a head that aligns them
+    constexpr int V = 16 / (int)sizeof(OT);
+    typedef OT vv __attribute__((ext_vector_type(V)));
+    const int nmat = count / N;
+    for (int j = 0; j < nmat; j++) {
+      OT* o = out + row_of<REG>(rows, j) * N;
+      const unsigned char* b = st8 + j * S * (int)sizeof(OT);
+      const int head = (V - (int)(((uintptr_t)o / sizeof(OT)) % V)) % V;
+      if (lane < head) o[lane] = lut_value<OT>(b, lut[lane]);
+      const int nv = (N - head) / V;
+      for (int v = lane; v < nv; v += WAVE) {
+        vv val;
+#pragma unroll
+        for (int q = 0; q < V; q++) val[q] = lut_value<OT>(b, lut[head + V * v + q]);
+        *reinterpret_cast<vv*>(o + head + V * v) = val;
+      }
+      const int t0 = head + V * nv;
+      if (lane < N - t0) o[t0 + lane] = lut_value<OT>(b, lut[t0 + lane]);
+    }
   } else {
     copy_out<N, VEC, REG>(out, rows, count, lane,
                           [&](int j, int e) { return lut_value<OT>(st8 + j * S * (int)sizeof(OT), lut[e]); });
