@@ -219,6 +219,7 @@ MF_DEV void gmf_series(const GmfSeriesIn& s, const double* __restrict__ bt, cpx*
   constexpr int ND = ORDER == 0 ? 3 : (ORDER == 1 ? 9 : 19);
   const bool inG[3] = {(s.flags & 1) != 0, (s.flags & 2) != 0, (s.flags & 4) != 0};
   const bool pk[3] = {(s.flags & 8) != 0, (s.flags & 16) != 0, (s.flags & 32) != 0};
+  const bool nopole = !(pk[0] || pk[1] || pk[2]);
   const double iap = s.iap, al = s.al;
   cpx E[ND];
 #pragma unroll
@@ -235,6 +236,8 @@ MF_DEV void gmf_series(const GmfSeriesIn& s, const double* __restrict__ bt, cpx*
                : bt[j + 1 < GMF_KMAX ? j + 1 : GMF_KMAX - 1];
     const double g1 = (bj - q1) * iap, g2 = (g1 - q2) * iap, g3 = (g2 - q3) * iap;
     q1 = g1; q2 = g2; q3 = g3;
+    // (without a pole every entry takes bj: the three factors set to bj once here leave two selects per entry)
+    const double h1 = nopole ? bj : g1, h2 = nopole ? bj : g2, h3 = nopole ? bj : g3;
     // E in place: children after their parents in the list, so a descending pass sees the parents' values at
     // j - 1 (a child whose node is in the group: parent(j-1) + u E(j-1)), and an ascending pass then copies the
     // parent's new value into a child whose node is outside the group.  (The descending pass updates every entry:
@@ -251,10 +254,10 @@ MF_DEV void gmf_series(const GmfSeriesIn& s, const double* __restrict__ bt, cpx*
       E[c] = pick(inG[i], E[c], par < 0 ? mk(0.0, 0.0) : E[par]);
       // pole factor: the multiplicity of the node outside a pair group selects G^(r) (masks loop-invariant,
       // indices constant)
-      const double qa = dd_a(c) == 0 ? bj : (dd_a(c) == 1 ? g1 : (dd_a(c) == 2 ? g2 : g3));
-      const double qb = dd_b(c) == 0 ? bj : (dd_b(c) == 1 ? g1 : (dd_b(c) == 2 ? g2 : g3));
-      const double qc = dd_c(c) == 0 ? bj : (dd_c(c) == 1 ? g1 : (dd_c(c) == 2 ? g2 : g3));
-      const double g = pk[0] ? qa : (pk[1] ? qb : (pk[2] ? qc : bj));
+      const double qa = dd_a(c) == 0 ? bj : (dd_a(c) == 1 ? h1 : (dd_a(c) == 2 ? h2 : h3));
+      const double qb = dd_b(c) == 0 ? bj : (dd_b(c) == 1 ? h1 : (dd_b(c) == 2 ? h2 : h3));
+      const double qc = dd_c(c) == 0 ? bj : (dd_c(c) == 1 ? h1 : (dd_c(c) == 2 ? h2 : h3));
+      const double g = pk[0] ? qa : (pk[1] ? qb : qc);
       acc[c] = add(acc[c], scale(E[c], g));
     }
   }
