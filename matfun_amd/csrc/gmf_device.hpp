@@ -213,35 +213,44 @@ struct GmfSeriesIn {
 };
 
 // acc[c] = the series sums of the ND = 3 / 9 / 19 entries of the divided-difference list (order 0 / 1 / 2), streamed
-// over the series index j; bt: the coefficients b_j (PA: 1 / j, the coefficients formed per lane)
+// over the series index j; bt: the coefficients b_j (PA: 1 / j, the coefficients formed per lane).
+// The dual vectors E follow one recurrence for every entry, E_j[c] = E_(j-1)[parent] + u_i E_(j-1)[c] (i the node
+// the entry adds; a descending pass reads the parents' old values), with the offset u of a node outside a pair group
+// zero.  An entry that adds the outside node then repeats its parent one term late, so every entry c runs
+// m(c) terms behind the group series it stands for (m = the multiplicity in c of the outside node, which is also
+// the pole node): term j of entry c is taken with the pole factor G^(m) of term j - m (kept for the last three
+// terms), and the loop runs ORDER + 1 terms past K with the factors of terms >= K zero.  Same values, sums and order
+// of operations as copying the parent's value into such entries at every term (up to the sign of exact zeros),
+// without the per-entry selects of the copy.
 template <int KIND, int ORDER, bool PA>
 MF_DEV void gmf_series(const GmfSeriesIn& s, const double* __restrict__ bt, cpx* acc) {
-  constexpr int ND = ORDER == 0 ? 3 : (ORDER == 1 ? 9 : 19);
-  const bool inG[3] = {(s.flags & 1) != 0, (s.flags & 2) != 0, (s.flags & 4) != 0};
+  constexpr int ND = ORDER == 0 ? 3 : (ORDER == 1 ? 9 : 19), LAG = ORDER + 1;
   const bool pk[3] = {(s.flags & 8) != 0, (s.flags & 16) != 0, (s.flags & 32) != 0};
   const bool nopole = !(pk[0] || pk[1] || pk[2]);
   const double iap = s.iap, al = s.al;
   cpx E[ND];
 #pragma unroll
   for (int c = 0; c < ND; c++) {
-    const int par = dd_parent(c);
-    E[c] = inG[dd_added(c)] ? mk(0.0, 0.0) : (par < 0 ? mk(1.0, 0.0) : E[par]);
+    E[c] = mk(0.0, 0.0);
     acc[c] = mk(0.0, 0.0);
   }
-  double q1 = 0.0, q2 = 0.0, q3 = 0.0;
+  double q1 = 0.0, q2 = 0.0, q3 = 0.0;        // G^(1..3) of the previous term
+  double r2 = 0.0, r3a = 0.0, r3b = 0.0;        // G^(2) of term j - 2, G^(3) of terms j - 2 and j - 3
   double bnext = 0.0;                           // (b_j loaded one iteration ahead of its use)
-  for (int j = 0; j < s.K; j++) {
-    const double bj = j == 0 ? s.b0 : bnext;
-    bnext = PA ? bj * ((al - j) * bt[j + 1 < GMF_KMAX ? j + 1 : GMF_KMAX - 1])
+  const int KT = s.K > 0 ? s.K + LAG : 0;
+  for (int j = 0; j < KT; j++) {
+    const double bl = j == 0 ? s.b0 : bnext;
+    bnext = PA ? bl * ((al - j) * bt[j + 1 < GMF_KMAX ? j + 1 : GMF_KMAX - 1])
                : bt[j + 1 < GMF_KMAX ? j + 1 : GMF_KMAX - 1];
-    const double g1 = (bj - q1) * iap, g2 = (g1 - q2) * iap, g3 = (g2 - q3) * iap;
+    // factors by the multiplicity m of the pole node: b_j, G^(1) of term j - 1, G^(2) of j - 2, G^(3) of j - 3
+    // (zero for terms past K - 1; without a pole every entry takes b_j: two selects per entry below)
+    const double bj = j < s.K ? bl : 0.0;
+    const double f1 = nopole ? bj : (j - 1 < s.K ? q1 : 0.0);
+    const double f2 = nopole ? bj : (j - 2 < s.K ? r2 : 0.0);
+    const double f3 = nopole ? bj : (j - 3 < s.K ? r3b : 0.0);
+    r3b = r3a; r3a = q3; r2 = q2;
+    const double g1 = (bl - q1) * iap, g2 = (g1 - q2) * iap, g3 = (g2 - q3) * iap;
     q1 = g1; q2 = g2; q3 = g3;
-    // (without a pole every entry takes bj: the three factors set to bj once here leave two selects per entry)
-    const double h1 = nopole ? bj : g1, h2 = nopole ? bj : g2, h3 = nopole ? bj : g3;
-    // E in place: children after their parents in the list, so a descending pass sees the parents' values at
-    // j - 1 (a child whose node is in the group: parent(j-1) + u E(j-1)), and an ascending pass then copies the
-    // parent's new value into a child whose node is outside the group.  (The descending pass updates every entry:
-    // an entry outside the group is read by its children before it is updated, and overwritten by the copy.)
 #pragma unroll
     for (int c = ND - 1; c >= 0; c--) {
       const int par = dd_parent(c), i = dd_added(c);
@@ -250,13 +259,10 @@ MF_DEV void gmf_series(const GmfSeriesIn& s, const double* __restrict__ bt, cpx*
     }
 #pragma unroll
     for (int c = 0; c < ND; c++) {
-      const int par = dd_parent(c), i = dd_added(c);
-      E[c] = pick(inG[i], E[c], par < 0 ? mk(0.0, 0.0) : E[par]);
-      // pole factor: the multiplicity of the node outside a pair group selects G^(r) (masks loop-invariant,
-      // indices constant)
-      const double qa = dd_a(c) == 0 ? bj : (dd_a(c) == 1 ? h1 : (dd_a(c) == 2 ? h2 : h3));
-      const double qb = dd_b(c) == 0 ? bj : (dd_b(c) == 1 ? h1 : (dd_b(c) == 2 ? h2 : h3));
-      const double qc = dd_c(c) == 0 ? bj : (dd_c(c) == 1 ? h1 : (dd_c(c) == 2 ? h2 : h3));
+      // (masks loop-invariant, indices constant)
+      const double qa = dd_a(c) == 0 ? bj : (dd_a(c) == 1 ? f1 : (dd_a(c) == 2 ? f2 : f3));
+      const double qb = dd_b(c) == 0 ? bj : (dd_b(c) == 1 ? f1 : (dd_b(c) == 2 ? f2 : f3));
+      const double qc = dd_c(c) == 0 ? bj : (dd_c(c) == 1 ? f1 : (dd_c(c) == 2 ? f2 : f3));
       const double g = pk[0] ? qa : (pk[1] ? qb : qc);
       acc[c] = add(acc[c], scale(E[c], g));
     }
