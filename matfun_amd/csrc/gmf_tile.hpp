@@ -147,8 +147,12 @@ MF_DEV void gmf_tile(const R& rows, int nm, const double* __restrict__ A, OT* __
 #pragma unroll
   for (int k = 0; k < 9; k++) xr[lane * 9 + k] = (OT)o.F[k];
   wave_sync();
-  if constexpr (R::RUN) copy_block<VEC>(F + rows.m(0) * 9, xr, 9 * nm, lane);
-  else copy_out<9, 1, R_IN>(F, rows, 9 * nm, lane, [&](int j, int e) { return xr[9 * j + e]; });
+  if constexpr (R::RUN) {
+    copy_block<VEC>(F + rows.m(0) * 9, xr, 9 * nm, lane);
+  } else {                                           // (rows listed: runs of consecutive rows as blocks)
+    for_row_runs<R_IN>(rows, nm, lane,
+                       [&](int j0, int j1) { copy_block<VEC>(F + rows.m(j0) * 9, xr + 9 * j0, 9 * (j1 - j0), lane); });
+  }
   if (ORDER >= 1) {
     wave_sync();
 #pragma unroll
@@ -183,8 +187,14 @@ MF_DEV void gmf_tile(const R& rows, int nm, const double* __restrict__ A, OT* __
         }
       }
       wave_sync();
-      if constexpr (R::RUN) copy_block<VEC>(J + rows.jr(p0) * 81, xr, 81 * cnt, lane);
-      else copy_out<81, 1, R_J>(J, shift_rows(rows, p0), 81 * cnt, lane, [&](int j, int e) { return xr[81 * j + e]; });
+      if constexpr (R::RUN) {
+        copy_block<VEC>(J + rows.jr(p0) * 81, xr, 81 * cnt, lane);
+      } else {
+        const R sub = shift_rows(rows, p0);
+        for_row_runs<R_J>(sub, cnt, lane, [&](int j0, int j1) {
+          copy_block<VEC>(J + sub.jr(j0) * 81, xr + 81 * j0, 81 * (j1 - j0), lane);
+        });
+      }
       wave_sync();
     }
     if (ORDER >= 2) {

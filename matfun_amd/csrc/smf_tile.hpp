@@ -130,7 +130,14 @@ MF_DEV void smf_tile(const R& rows, int nm, const double* __restrict__ x, OT* __
     }
   }
   __syncthreads();
-  copy_out<NF, 1, R_IN>(F, rows, NF * nm, lane, [&](int j, int e) { return t1[j * SA + f_src(MAT, e)]; });
+  if constexpr (R::RUN) {
+    copy_out<NF, 1, R_IN>(F, rows, NF * nm, lane, [&](int j, int e) { return t1[j * SA + f_src(MAT, e)]; });
+  } else {                                     // (rows listed: runs of consecutive rows as blocks)
+    for_row_runs<R_IN>(rows, nm, lane, [&](int j0, int j1) {
+      copy_run<NF>(F + rows.m(j0) * NF, (j1 - j0) * NF, lane,
+                   [&](int j, int e) { return t1[(j0 + j) * SA + f_src(MAT, e)]; });
+    });
+  }
   if (ORDER >= 1) {
     if constexpr (PACK)
       copy_out<NJ, VEC, R_J>(J, rows, NJ * nm, lane, [&](int j, int e) { return t1[j * SA + 6 + e]; });

@@ -123,6 +123,51 @@ MF_DEV void copy_out(OT* __restrict__ out, const R& rows, int count, int lane, G
   }
 }
 
+// the maximal runs of consecutive rows (region REG) among the tile matrices [0, cnt): fn(j0, j1) for each run
+// [j0, j1), in order, by the whole wave (the runs come from a ballot: uniform).  Rows listed per matrix are mostly
+// consecutive (a batch's groups of matrices), and a run's outputs form one contiguous block.
+template <int REG, class R, class Fn>
+__device__ void for_row_runs(const R& rows, int cnt, int lane, Fn fn) {
+  const bool c = lane > 0 && lane < cnt && row_of<REG>(rows, lane) == row_of<REG>(rows, lane - 1) + 1;
+  const uint64_t cont = __ballot(c);
+  for (int j0 = 0; j0 < cnt;) {
+    const uint64_t stop = j0 + 1 < 64 ? (~cont & (~0ull << (j0 + 1))) : 0ull;
+    int j1 = stop ? __ffsll((long long)stop) - 1 : 64;
+    j1 = j1 < cnt ? j1 : cnt;
+    fn(j0, j1);
+    j0 = j1;
+  }
+}
+
+// o[t] = get(t / N, t % N) for t < count (one run's block), by the wave, in stores of 16 bytes aligned to 16 bytes:
+// the head (the elements before the first aligned vector) and the tail one element per lane; o needs only the
+// alignment of OT
+template <int N, class OT, class Get>
+__device__ void copy_run(OT* __restrict__ o, int count, int lane, Get get) {
+  constexpr int V = 16 / (int)sizeof(OT);
+  typedef OT vv __attribute__((ext_vector_type(V)));
+  const int mis = (int)(((uintptr_t)o / sizeof(OT)) % V);
+  const int head = (V - mis) % V < count ? (V - mis) % V : count;
+  if (lane < head) o[lane] = get(lane / N, lane % N);
+  int t = head + V * lane, j = t / N, e = t - j * N;
+  for (; t + V - 1 < count; t += V * WAVE) {
+    vv val;
+    int jq = j, eq = e;
+#pragma unroll
+    for (int q = 0; q < V; q++) {
+      val[q] = get(jq, eq);
+      const bool wrap = eq + 1 == N;
+      eq = wrap ? 0 : eq + 1;
+      jq = wrap ? jq + 1 : jq;
+    }
+    *reinterpret_cast<vv*>(o + t) = val;
+    e += V * WAVE;
+    while (e >= N) { e -= N; j++; }
+  }
+  const int t0 = head + (count - head) / V * V;
+  if (lane < count - t0) o[t0 + lane] = get((t0 + lane) / N, (t0 + lane) % N);
+}
+
 // Output layouts through a lookup table: element e of a matrix's output row is a value staged in LDS (the matrix's
 // block: S elements per matrix) times a power of two.  Entry lut[e] (a short): bits 0..11 the byte offset of that
 // value from the matrix's block, bits 12..15 the exponent x (signed: -8 .. 7); out = value 2^x.  Entries N .. N +
@@ -138,6 +183,34 @@ MF_DEV OT lut_value(const unsigned char* blk, int u) {       // (ldexp: the exac
   const OT v = *reinterpret_cast<const OT*>(blk + (u & 0xfff));
   if constexpr (sizeof(OT) == 4) return __builtin_ldexpf(v, u >> 12);
   else return __builtin_ldexp(v, u >> 12);
+}
+
+// one run's block through the lookup table: o[t] = lut_value(block of run matrix t / N, lut[t % N]), blocks at st8 +
+// (t / N) S elements; 16-byte stores after an aligned head (as copy_run), the table's padding for vectors across a
+// matrix boundary (as copy_out_lut)
+template <int N, int S, class OT>
+__device__ void copy_run_lut(OT* __restrict__ o, int count, int lane, const unsigned char* st8, const short* lut) {
+  constexpr int V = 16 / (int)sizeof(OT);
+  static_assert(V - 1 <= LUT_PAD && V <= N, "vector longer than the table's padding or a row");
+  typedef OT vv __attribute__((ext_vector_type(V)));
+  const int mis = (int)(((uintptr_t)o / sizeof(OT)) % V);
+  const int head = (V - mis) % V < count ? (V - mis) % V : count;
+  if (lane < head) o[lane] = lut_value<OT>(st8, lut[lane]);          // (head < V <= N: the first matrix)
+  int t = head + V * lane, j = t / N, e = t - j * N;
+  const unsigned char* b = st8 + j * S * (int)sizeof(OT);
+  for (; t + V - 1 < count; t += V * WAVE) {
+    vv val;
+#pragma unroll
+    for (int q = 0; q < V; q++) val[q] = lut_value<OT>(b, lut[e + q]);
+    *reinterpret_cast<vv*>(o + t) = val;
+    e += V * WAVE;
+    while (e >= N) { e -= N; b += S * (int)sizeof(OT); }
+  }
+  const int t0 = head + (count - head) / V * V;
+  if (lane < count - t0) {
+    const int tt = t0 + lane, jj = tt / N;
+    o[tt] = lut_value<OT>(st8 + jj * S * (int)sizeof(OT), lut[tt - jj * N]);
+  }
 }
 
 // copy_out of a layout with a lookup table: out[row(j) N + e] = lut_value(staged block of matrix j, lut[e]), blocks
@@ -173,26 +246,12 @@ MF_DEV void copy_out_lut(OT* __restrict__ out, const R& rows, int count, int lan
       blk[t] = lut_value<OT>(b, lut[e]);
       if (++e == N) { e = 0; b += S * (int)sizeof(OT); }
     }
-  } else if constexpr (!R::RUN && N >= 4 * WAVE) {
-    // rows listed, long rows (H): each matrix's row by itself, in 16-byte stores after a head that aligns them
-    constexpr int V = 16 / (int)sizeof(OT);
-    typedef OT vv __attribute__((ext_vector_type(V)));
-    const int nmat = count / N;
-    for (int j = 0; j < nmat; j++) {
-      OT* o = out + row_of<REG>(rows, j) * N;
-      const unsigned char* b = st8 + j * S * (int)sizeof(OT);
-      const int head = (V - (int)(((uintptr_t)o / sizeof(OT)) % V)) % V;
-      if (lane < head) o[lane] = lut_value<OT>(b, lut[lane]);
-      const int nv = (N - head) / V;
-      for (int v = lane; v < nv; v += WAVE) {
-        vv val;
-#pragma unroll
-        for (int q = 0; q < V; q++) val[q] = lut_value<OT>(b, lut[head + V * v + q]);
-        *reinterpret_cast<vv*>(o + head + V * v) = val;
-      }
-      const int t0 = head + V * nv;
-      if (lane < N - t0) o[t0 + lane] = lut_value<OT>(b, lut[t0 + lane]);
-    }
+  } else if constexpr (!R::RUN) {
+    // rows listed: each run of consecutive rows as one block (a run of one matrix: its row alone)
+    for_row_runs<REG>(rows, count / N, lane, [&](int j0, int j1) {
+      copy_run_lut<N, S, OT>(out + row_of<REG>(rows, j0) * N, (j1 - j0) * N, lane, st8 + j0 * S * (int)sizeof(OT),
+                             lut);
+    });
   } else {
     copy_out<N, VEC, REG>(out, rows, count, lane,
                           [&](int j, int e) { return lut_value<OT>(st8 + j * S * (int)sizeof(OT), lut[e]); });
