@@ -712,51 +712,47 @@ MF_DEV void gmf_j_lane(const T* r1, const T* r2, const T* d2s, T sc1, int b, int
 }
 
 // H_abcdef = sc2 (T6_acdefb + T6_aefcdb), T6_acdefb = sum_mnp D3_mnp r_m[a][c] r_n[d][e] r_p[f][b]
-// (gmf_kernel.hpp:672-691), for lane (d, e, f) and (a, b, c) = 0..26: with Z[mn][fb] = sum_p D3_mnp r_p[f][b] and
-// Yh[m][(de)(fb)] = sum_n r_n[d][e] Z[mn][fb], H = sc2 (sum_m r_m[a][c] X1[m][b] + sum_m r_m[a][e] X2[m][c][b]),
-// X1[m][b] = Yh[m][(de)(fb)], X2[m][c][b] = Yh[m][(fc)(db)]; out[9a + 3b + c]
+// (gmf_kernel.hpp:672-691), assembled by 27 lanes (d, e, f) per matrix in two steps.  With Z_mn[fb] = sum_p D3_mnp
+// r_p[f][b] and Y_m[(de)(fb)] = sum_n r_n[d][e] Z_mn[fb]: H = sc2 (sum_m r_m[a][c] X1[m][b] + sum_m r_m[a][e]
+// X2[m][c][b]) with X1[m][b] = Y_m[(de)(fb)] and X2[m][c][b] = Y_m[(fc)(db)] -- the X1 of lane (f, c, d).
+// gmf_h_y: the lane's X1 (y[3 m + b]); gmf_h_out: its 27 outputs out[9a + 3b + c] from its X1 and the 27 lanes' X1
+// (ys: lane L's at ys[9 L])
 template <class T>
-MF_DEV void gmf_h_lane(const T* r1, const T* r2, const T* d3s, T sc2, int d, int e, int f, T* out) {
-  T rfb[3][3], rdb[3][3], rde[3];   // r_p[f][b], r_p[d][b], r_p[d][e]
+MF_DEV void gmf_h_y(const T* r1, const T* r2, const T* d3s, int d, int e, int f, T* y) {
+  T rfb[3][3], rde[3];              // r_p[f][b], r_p[d][e]
 #pragma unroll
   for (int p = 0; p < 3; p++) {
     rde[p] = rget(r1, r2, p, d, e);
 #pragma unroll
-    for (int b = 0; b < 3; b++) {
-      rfb[p][b] = rget(r1, r2, p, f, b);
-      rdb[p][b] = rget(r1, r2, p, d, b);
-    }
+    for (int b = 0; b < 3; b++) rfb[p][b] = rget(r1, r2, p, f, b);
   }
-  T X1[3][3], X2[3][3][3];
 #pragma unroll
   for (int mm = 0; mm < 3; mm++) {
-    T Z1[3][3], Z2[3][3];        // [n][b]: Z[mn][fb], Z[mn][db]
+    T Z1[3][3];                     // [n][b]: Z[mn][fb]
 #pragma unroll
     for (int nn = 0; nn < 3; nn++)
 #pragma unroll
       for (int b = 0; b < 3; b++) {
         const T D0 = d3s[sym33(mm, nn, 0)], D1 = d3s[sym33(mm, nn, 1)], D2 = d3s[sym33(mm, nn, 2)];
         Z1[nn][b] = D0 * rfb[0][b] + D1 * rfb[1][b] + D2 * rfb[2][b];
-        Z2[nn][b] = D0 * rdb[0][b] + D1 * rdb[1][b] + D2 * rdb[2][b];
       }
 #pragma unroll
-    for (int b = 0; b < 3; b++) {
-      X1[mm][b] = rde[0] * Z1[0][b] + rde[1] * Z1[1][b] + rde[2] * Z1[2][b];
-#pragma unroll
-      for (int c = 0; c < 3; c++)
-        X2[mm][c][b] = rfb[0][c] * Z2[0][b] + rfb[1][c] * Z2[1][b] + rfb[2][c] * Z2[2][b];
-    }
+    for (int b = 0; b < 3; b++) y[3 * mm + b] = rde[0] * Z1[0][b] + rde[1] * Z1[1][b] + rde[2] * Z1[2][b];
   }
+}
+template <class T>
+MF_DEV void gmf_h_out(const T* r1, const T* r2, T sc2, int d, int e, int f, const T* x1, const T* ys, T* out) {
 #pragma unroll
   for (int aa = 0; aa < 3; aa++) {
     const T re0 = rget(r1, r2, 0, aa, e), re1 = rget(r1, r2, 1, aa, e), re2 = rget(r1, r2, 2, aa, e);
 #pragma unroll
     for (int c = 0; c < 3; c++) {
       const T rc0 = rget(r1, r2, 0, aa, c), rc1 = rget(r1, r2, 1, aa, c), rc2 = rget(r1, r2, 2, aa, c);
+      const T* x2 = ys + 9 * (9 * f + 3 * c + d);   // X2[m][c][b] = x2[3 m + b]
 #pragma unroll
       for (int b = 0; b < 3; b++)
-        out[9 * aa + 3 * b + c] = (rc0 * X1[0][b] + rc1 * X1[1][b] + rc2 * X1[2][b] + re0 * X2[0][c][b] +
-                                   re1 * X2[1][c][b] + re2 * X2[2][c][b]) * sc2;
+        out[9 * aa + 3 * b + c] = (rc0 * x1[b] + rc1 * x1[3 + b] + rc2 * x1[6 + b] + re0 * x2[b] + re1 * x2[3 + b] +
+                                   re2 * x2[6 + b]) * sc2;
     }
   }
 }

@@ -202,12 +202,19 @@ MF_DEV void gmf_tile(const R& rows, int nm, const double* __restrict__ A, OT* __
       const int g = lane / G, Lg = lane - g * G;
       for (int p0 = 0; p0 < nm; p0 += MPP) {        // H: H_abcdef, (a,b,c) = 0..26 at 27 (9a + 3b + c) + (9d + 3e + f)
         const int cnt = nm - p0 < MPP ? nm - p0 : MPP;
-        OT out[27];
         const OT s2 = __shfl(sc2, (p0 + g) & (WAVE - 1));   // (sc2 of the lane group's matrix, from its lane)
+        const OT* s = mainr + (p0 + g) * L::MS;
+        OT* ys = xr + g * 243;                         // the 27 lanes' X1 (gmf_h_y), in the H tile's space
+        const int d = Lg / 9, e = (Lg / 3) % 3, f = Lg % 3;
+        OT y[9], out[27];
         if (g < cnt) {
-          const OT* s = mainr + (p0 + g) * L::MS;
-          gmf_h_lane(s, s + 9, s + 18, s2, Lg / 9, (Lg / 3) % 3, Lg % 3, out);
+          gmf_h_y(s, s + 9, s + 18, d, e, f, y);
+#pragma unroll
+          for (int k = 0; k < 9; k++) ys[9 * Lg + k] = y[k];
         }
+        wave_sync();
+        if (g < cnt) gmf_h_out(s, s + 9, s2, d, e, f, y, ys, out);
+        wave_sync();                                   // (X1 read: the H tile takes the outputs)
         for (int gg = 0; gg < cnt; gg++) {         // one matrix at a time: its 729 values staged, then stored
           if (g == gg) {
 #pragma unroll

@@ -85,8 +85,10 @@ void gmf_all(double al, const double* alv, int64_t n, const double* A, double* F
     for (int i = 0; i < 3; i++)
       for (int j = i; j < 3; j++)
         for (int k = j; k < 3; k++) d3s[mf::sym33s(i, j, k)] = o.D3[9 * i + 3 * j + k];
+    double ys[27 * 9];
+    for (int L = 0; L < 27; L++) mf::gmf_h_y(r1, r2, d3s, L / 9, (L / 3) % 3, L % 3, ys + 9 * L);
     for (int L = 0; L < 27; L++) {
-      mf::gmf_h_lane(r1, r2, d3s, o.sc2, L / 9, (L / 3) % 3, L % 3, out);
+      mf::gmf_h_out(r1, r2, o.sc2, L / 9, (L / 3) % 3, L % 3, ys + 9 * L, ys, out);
       for (int k = 0; k < 27; k++) H[729 * m + 27 * k + L] = out[k];
     }
   }
