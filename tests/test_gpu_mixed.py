@@ -46,3 +46,29 @@ def test_gmf_mixed(oracle, kind, alpha, order):
     for name, x, y in (("F", F, Fo), ("J", J, Jo), ("H", H, Ho))[:order + 1]:
         e = normwise(x, y)
         assert e.max() <= TOL[name], (kind, alpha, name, e.max(), np.quantile(e, 0.999))
+
+
+@pytest.mark.parametrize("n", [1, 63, 64, 65, 191, 255, 256, 257, 1000])
+def test_gmf_mixed_order1_shared_series_equals_own_series(n):
+    """The FP32 order-1 gmf kernels of log and powers run four tiles per workgroup and deal the group series out by
+    length across the workgroup's lanes; a heterogeneous batch of one group computes every series in the lane of its
+    matrix.  Same arithmetic, so bitwise the same F and J -- at sizes around the 64-matrix tile and the 256-matrix
+    workgroup (wavefronts of a workgroup past the end take part with no matrices)."""
+    import torch
+    import matfun_amd
+    from tests.gpu_util import call, call_pow
+    G = W.general(n, seed=505 + n)
+    for kind, alpha in (("log", 0.0), ("sqrt", 0.0), ("pow", 0.3)):
+        # (a batch's pow groups take per-matrix exponents: the counterpart is the per-matrix entry point)
+        if kind == "pow":
+            rc, F, J, _ = call_pow("general", np.full(n, alpha), G, 1, mixed=True)
+        else:
+            rc, F, J, _ = call("general", kind, alpha, G, 1, mixed=True)
+        assert rc == 0
+        b = matfun_amd.Batch([("gmf", kind, 1, n, alpha)])
+        Fb, Jb, _ = b.eval(torch.from_numpy(G).cuda(), precision="mixed")
+        torch.cuda.synchronize()
+        for name, x, y in (("F", F, Fb), ("J", J, Jb)):
+            y = y.reshape(n, -1).cpu().double().numpy()
+            assert np.array_equal(x.reshape(n, -1), y, equal_nan=True), (kind, n, name)
+        b.close()
