@@ -3,7 +3,8 @@
 //             of tile matrix j are all base + j, so every region a tile writes is one contiguous block
 //   ListRows  rows listed per matrix (heterogeneous batches): input / F row m[j], J row jr[j], H row hr[j], copied
 //             into LDS at the start of the tile
-// The tile bodies (smf_tile.hpp, gmf_tile.hpp) take either; with RunRows they compile to the contiguous addressing.
+// The tile bodies (smf_tile.hpp, gmf_tile.hpp) take either; with RunRows they compile to the contiguous addressing,
+// with ListRows each run of consecutive rows (a batch's groups) is written as one block.
 #pragma once
 #include "mf_device.hpp"
 
@@ -62,15 +63,16 @@ MF_DEV int64_t row_of(const R& r, int j) {
 }
 
 // out[row(j) N + e] = get(j, e) for the count = N nm elements (t = j N + e) of the tile's matrices j = 0 .. nm-1 in
-// region REG, by the wave.  RunRows: one contiguous block (out + base N + t); with VEC > 1, VEC consecutive elements
-// per lane and one vector store (VEC x sizeof(OT) bytes: 16 B for FP64 pairs and FP32 quadruples; the block must be
-// aligned to it), the last count % VEC elements one at a time; else one element per lane.  ListRows: one element
-// per lane, per-matrix row from LDS.
+// region REG, by the wave, for consecutive rows (RunRows: one contiguous block, out + base N + t; rows listed per
+// matrix take for_row_runs with copy_run / copy_run_lut).  With VEC > 1, VEC consecutive elements per lane and one
+// vector store (VEC x sizeof(OT) bytes: 16 B for FP64 pairs and FP32 quadruples; the block must be aligned to it),
+// the last count % VEC elements one at a time; else one element per lane.
 template <int N, int VEC, int REG, class R, class OT, class Get>
 MF_DEV void copy_out(OT* __restrict__ out, const R& rows, int count, int lane, Get get) {
-  if (R::RUN && VEC > 1) {
+  static_assert(R::RUN, "copy_out writes consecutive rows");
+  OT* blk = out + row_of<REG>(rows, 0) * N;
+  if constexpr (VEC > 1) {
     typedef OT vv __attribute__((ext_vector_type(VEC)));
-    OT* blk = out + row_of<REG>(rows, 0) * N;
     int t = VEC * lane, j = t / N, e = t - j * N;
     for (; t + VEC - 1 < count; t += VEC * WAVE) {
       vv val;
@@ -90,33 +92,10 @@ MF_DEV void copy_out(OT* __restrict__ out, const R& rows, int count, int lane, G
       blk[t] = get(j, e);
       if (++e == N) { e = 0; j++; }
     }
-  } else if (R::RUN) {
-    OT* blk = out + row_of<REG>(rows, 0) * N;
-    int j = lane / N, e = lane - j * N;
-    for (int t = lane; t < count; t += WAVE) {
-      blk[t] = get(j, e);
-      e += WAVE;
-      while (e >= N) { e -= N; j++; }
-    }
-  } else if (N >= WAVE) {
-    // rows listed, N >= 64: a running pointer per lane, reloaded when the lane passes to the next matrix (at most
-    // once per step; only for an element that exists, so only rows of the tile's matrices are read)
-    int j = 0, e = lane;
-    OT* p = out + row_of<REG>(rows, 0) * N + lane;
-    for (int t = lane; t < count; t += WAVE) {
-      if (e >= N) {
-        e -= N;
-        j++;
-        p = out + row_of<REG>(rows, j) * N + e;
-      }
-      *p = get(j, e);
-      e += WAVE;
-      p += WAVE;
-    }
   } else {
     int j = lane / N, e = lane - j * N;
     for (int t = lane; t < count; t += WAVE) {
-      out[row_of<REG>(rows, j) * N + e] = get(j, e);
+      blk[t] = get(j, e);
       e += WAVE;
       while (e >= N) { e -= N; j++; }
     }
