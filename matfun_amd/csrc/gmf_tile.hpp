@@ -75,7 +75,8 @@ struct GmfSeriesShared {
 // LDS layout (elements of OT):
 //   main [64][MS]   r1 (9), r2 (9); order 2: the 10 distinct D3 (MS = 28; order 1: 19)
 //   x    [XS]       phase 1: F of the 64 matrices [0, 576), then D2 (6 distinct) and sc1 per matrix [XAUX, +448);
-//                   J passes: the JMPP x 81 output tile [0, XAUX); H passes: the 729 values of one matrix
+//                   J passes: the JMPP x 81 output tile [0, XAUX); H passes: the 729 values of one matrix (FP32:
+//                   of both matrices of the pass, HT)
 // (the FP64 input block of 64 x 9 doubles overlays the start, before phase 1)
 template <int ORDER, class OT>
 struct GmfLds {
@@ -83,7 +84,8 @@ struct GmfLds {
   static constexpr int JG = ORDER == 1 ? 9 : 27;            // J: lanes per matrix
   static constexpr int JMPP = WAVE / JG;                    // J: matrices per pass
   static constexpr int XAUX = JMPP * 81;
-  static constexpr int XS = ORDER == 0 ? 576 : (ORDER == 1 || XAUX + 448 > 729 ? XAUX + 448 : 729);
+  static constexpr int HT = sizeof(OT) == 4 ? 1458 : 729;  // H passes: both matrices (FP32), else one at a time
+  static constexpr int XS = ORDER == 0 ? 576 : (ORDER == 1 || XAUX + 448 > HT ? XAUX + 448 : HT);
   static constexpr int MAIN = ORDER == 0 ? 0 : 64 * MS;
   static constexpr int TOT = MAIN + XS;
   static constexpr int BYTES = (TOT * (int)sizeof(OT) > WAVE * 9 * 8) ? TOT * (int)sizeof(OT) : WAVE * 9 * 8;
@@ -215,14 +217,24 @@ MF_DEV void gmf_tile(const R& rows, int nm, const double* __restrict__ A, OT* __
         wave_sync();
         if (g < cnt) gmf_h_out(s, s + 9, s2, d, e, f, y, ys, out);
         wave_sync();                                   // (X1 read: the H tile takes the outputs)
-        for (int gg = 0; gg < cnt; gg++) {         // one matrix at a time: its 729 values staged, then stored
-          if (g == gg) {
+        if constexpr (L::HT == 1458 && R::RUN) {   // both matrices staged, stored as one block (consecutive rows)
+          if (g < cnt) {
 #pragma unroll
-            for (int k = 0; k < 27; k++) xr[27 * k + Lg] = out[k];
+            for (int k = 0; k < 27; k++) xr[g * 729 + 27 * k + Lg] = out[k];
           }
           wave_sync();
-          copy_block<VEC>(H + rows.hr(p0 + gg) * 729, xr, 729, lane);
+          copy_block<VEC>(H + rows.hr(p0) * 729, xr, 729 * cnt, lane);
           wave_sync();
+        } else {
+          for (int gg = 0; gg < cnt; gg++) {       // one matrix at a time: its 729 values staged, then stored
+            if (g == gg) {
+#pragma unroll
+              for (int k = 0; k < 27; k++) xr[27 * k + Lg] = out[k];
+            }
+            wave_sync();
+            copy_block<VEC>(H + rows.hr(p0 + gg) * 729, xr, 729, lane);
+            wave_sync();
+          }
         }
       }
     }
