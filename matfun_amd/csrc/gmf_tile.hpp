@@ -3,9 +3,10 @@
 //            r_m (27 values) and D3 (27) staged in LDS per matrix, D2 (9) and F (9) in a second region
 //   phase 2, lane groups, assembling J and H into an LDS tile in the API layout that the wave then writes as one
 //            contiguous block (16-byte stores for consecutive rows, copy_block): J with 9 lanes per matrix (9 values
-//            each), 7 matrices per pass (order 1) or 27 lanes per matrix (3 values each), 2 matrices per pass (order
-//            2, where the smaller J tile keeps the LDS of a wavefront under 20 KB: eight per CU in FP64); H with 27
-//            lanes per matrix (27 values each), 2 matrices per pass, stored one matrix (5.8 KB FP64) at a time
+//            each), 7 matrices per pass (order 1; FP32 order 2) or 27 lanes per matrix (3 values each), 2 matrices
+//            per pass (FP64 order 2, where the smaller J tile keeps the LDS of a wavefront under 20 KB: eight per
+//            CU); H with 27 lanes per matrix (27 values each), 2 matrices per pass, stored one matrix (5.8 KB FP64)
+//            at a time (FP32: both)
 // OT = double: the FP64 form; OT = float: the mixed form (FP64 per-lane math, FP32 staged tensors, J / H assembly
 // and outputs).  PA: powers with one exponent per matrix (alv[i], read by its lane; bt then holds 1 / j; a matrix
 // whose exponent is not finite gets NaN results).  Used by the homogeneous kernels (gmf_kernels.hip, RunRows) and the
@@ -81,7 +82,8 @@ struct GmfSeriesShared {
 template <int ORDER, class OT>
 struct GmfLds {
   static constexpr int MS = ORDER >= 2 ? 28 : 19;
-  static constexpr int JG = ORDER == 1 ? 9 : 27;            // J: lanes per matrix
+  static constexpr int JG = ORDER == 1 || sizeof(OT) == 4 ? 9 : 27;   // J: lanes per matrix (FP64 order 2: 27, the
+                                                                     // smaller J tile fits eight wavefronts per CU)
   static constexpr int JMPP = WAVE / JG;                    // J: matrices per pass
   static constexpr int XAUX = JMPP * 81;
   static constexpr int HT = sizeof(OT) == 4 ? 1458 : 729;  // H passes: both matrices (FP32), else one at a time
