@@ -3,10 +3,8 @@
 //            r_m (27 values) and D3 (27) staged in LDS per matrix, D2 (9) and F (9) in a second region
 //   phase 2, lane groups, assembling J and H into an LDS tile in the API layout that the wave then writes as one
 //            contiguous block (16-byte stores for consecutive rows, copy_block): J with 9 lanes per matrix (9 values
-//            each), 7 matrices per pass (order 1; FP32 order 2) or 27 lanes per matrix (3 values each), 2 matrices
-//            per pass (FP64 order 2, where the smaller J tile keeps the LDS of a wavefront under 20 KB: eight per
-//            CU); H with 27 lanes per matrix (27 values each), 2 matrices per pass, stored one matrix (5.8 KB FP64)
-//            at a time (FP32: both)
+//            each), 7 matrices per pass; H with 27 lanes per matrix (27 values each), 2 matrices per pass, stored one
+//            matrix (5.8 KB FP64) at a time (FP32: both)
 // OT = double: the FP64 form; OT = float: the mixed form (FP64 per-lane math, FP32 staged tensors, J / H assembly
 // and outputs).  PA: powers with one exponent per matrix (alv[i], read by its lane; bt then holds 1 / j; a matrix
 // whose exponent is not finite gets NaN results).  Used by the homogeneous kernels (gmf_kernels.hip, RunRows) and the
@@ -75,19 +73,23 @@ struct GmfSeriesShared {
 
 // LDS layout (elements of OT):
 //   main [64][MS]   r1 (9), r2 (9); order 2: the 10 distinct D3 (MS = 28; order 1: 19)
-//   x    [XS]       phase 1: F of the 64 matrices [0, 576), then D2 (6 distinct) and sc1 per matrix [XAUX, +448);
+//   x    [XS]       phase 1: F of the 64 matrices [0, 576), then D2 (6 distinct) and sc1 per matrix [XAUX, +448)
+//                   (not for FP64 order 2: AUXR);
 //                   J passes: the JMPP x 81 output tile [0, XAUX); H passes: the 729 values of one matrix (FP32:
 //                   of both matrices of the pass, HT)
 // (the FP64 input block of 64 x 9 doubles overlays the start, before phase 1)
 template <int ORDER, class OT>
 struct GmfLds {
   static constexpr int MS = ORDER >= 2 ? 28 : 19;
-  static constexpr int JG = ORDER == 1 || sizeof(OT) == 4 ? 9 : 27;   // J: lanes per matrix (FP64 order 2: 27, the
-                                                                     // smaller J tile fits eight wavefronts per CU)
-  static constexpr int JMPP = WAVE / JG;                    // J: matrices per pass
+  static constexpr int JG = 9;                              // J: lanes per matrix
+  static constexpr int JMPP = WAVE / JG;                    // J: matrices per pass (7)
   static constexpr int XAUX = JMPP * 81;
+  // FP64 order 2: D2 and sc1 reach the J lanes from their matrix's lane by shuffles (no aux region: the tile stays
+  // under 20 KB, eight wavefronts per CU); else staged in LDS after the J tile
+  static constexpr bool AUXR = ORDER == 2 && sizeof(OT) == 8;
   static constexpr int HT = sizeof(OT) == 4 ? 1458 : 729;  // H passes: both matrices (FP32), else one at a time
-  static constexpr int XS = ORDER == 0 ? 576 : (ORDER == 1 || XAUX + 448 > HT ? XAUX + 448 : HT);
+  static constexpr int XJ = XAUX + (AUXR ? 0 : 448);
+  static constexpr int XS = ORDER == 0 ? 576 : (ORDER == 1 || XJ > HT ? XJ : HT);
   static constexpr int MAIN = ORDER == 0 ? 0 : 64 * MS;
   static constexpr int TOT = MAIN + XS;
   static constexpr int BYTES = (TOT * (int)sizeof(OT) > WAVE * 9 * 8) ? TOT * (int)sizeof(OT) : WAVE * 9 * 8;
@@ -158,36 +160,43 @@ MF_DEV void gmf_tile(const R& rows, int nm, const double* __restrict__ A, OT* __
                        [&](int j0, int j1) { copy_block<VEC>(F + rows.m(j0) * 9, xr + 9 * j0, 9 * (j1 - j0), lane); });
   }
   if (ORDER >= 1) {
-    wave_sync();
+    OT d2r[7];                                       // D2 (6 distinct, by sym3) and sc1 of this lane's matrix
 #pragma unroll
     for (int i = 0; i < 3; i++)
 #pragma unroll
-      for (int j = i; j < 3; j++) xr[L::XAUX + lane * 7 + sym3(i, j)] = (OT)o.D2[3 * i + j];
-    xr[L::XAUX + lane * 7 + 6] = (OT)o.sc1;
-    wave_sync();
+      for (int j = i; j < 3; j++) d2r[sym3(i, j)] = (OT)o.D2[3 * i + j];
+    d2r[6] = (OT)o.sc1;
+    if constexpr (!L::AUXR) {
+      wave_sync();
+#pragma unroll
+      for (int k = 0; k < 7; k++) xr[L::XAUX + lane * 7 + k] = d2r[k];
+      wave_sync();
+    }
 
-    // ---- phase 2, J: lane (gj, Lj) = matrix gj of the pass, Lj = 9 b + 3 c + d (27 lanes: J_abcd for a = 0..2) or
-    //      Lj = 3 c + d (9 lanes: J_abcd for a, b = 0..2), each value at 27 a + 9 b + 3 c + d of its matrix
+    // ---- phase 2, J: lane (gj, Lj) = matrix gj of the pass, Lj = 3 c + d: J_abcd for a, b = 0..2, each value at
+    //      27 a + 9 b + 3 c + d of its matrix
     constexpr int JG = L::JG, JMPP = L::JMPP;
     const int gj = lane / JG, Lj = lane - gj * JG;
     for (int p0 = 0; p0 < nm; p0 += JMPP) {
       const int cnt = nm - p0 < JMPP ? nm - p0 : JMPP;
+      OT dj[7];
+      if constexpr (L::AUXR) {                       // (from the matrix's lane; every lane takes part)
+#pragma unroll
+        for (int k = 0; k < 7; k++) dj[k] = __shfl(d2r[k], (p0 + (gj < JMPP ? gj : 0)) & (WAVE - 1));
+      }
       if (gj < cnt) {
         const int j = p0 + gj;
         const OT* s = mainr + j * L::MS;
-        const OT* aux = xr + L::XAUX + j * 7;
+        if constexpr (!L::AUXR) {
+#pragma unroll
+          for (int k = 0; k < 7; k++) dj[k] = xr[L::XAUX + j * 7 + k];
+        }
         OT out[3];
-        if (JG == 27) {
-          gmf_j_lane(s, s + 9, aux, aux[6], Lj / 9, (Lj / 3) % 3, Lj % 3, out);
 #pragma unroll
-          for (int k = 0; k < 3; k++) xr[gj * 81 + 27 * k + Lj] = out[k];
-        } else {
+        for (int b = 0; b < 3; b++) {
+          gmf_j_lane(s, s + 9, dj, dj[6], b, Lj / 3, Lj % 3, out);
 #pragma unroll
-          for (int b = 0; b < 3; b++) {
-            gmf_j_lane(s, s + 9, aux, aux[6], b, Lj / 3, Lj % 3, out);
-#pragma unroll
-            for (int k = 0; k < 3; k++) xr[gj * 81 + 27 * k + 9 * b + Lj] = out[k];
-          }
+          for (int k = 0; k < 3; k++) xr[gj * 81 + 27 * k + 9 * b + Lj] = out[k];
         }
       }
       wave_sync();
