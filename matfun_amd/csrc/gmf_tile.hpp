@@ -74,7 +74,7 @@ struct GmfSeriesShared {
 // LDS layout (elements of OT):
 //   main [64][MS]   r1 (9), r2 (9); order 2: the 10 distinct D3 (MS = 28; order 1: 19)
 //   x    [XS]       phase 1: F of the 64 matrices [0, 576), then D2 (6 distinct) and sc1 per matrix [XAUX, +448)
-//                   (not for FP64 order 2: AUXR);
+//                   (FP32 only: AUXR);
 //                   J passes: the JMPP x 81 output tile [0, XAUX); H passes: the 729 values of one matrix (FP32:
 //                   of both matrices of the pass, HT)
 // (the FP64 input block of 64 x 9 doubles overlays the start, before phase 1)
@@ -84,12 +84,13 @@ struct GmfLds {
   static constexpr int JG = 9;                              // J: lanes per matrix
   static constexpr int JMPP = WAVE / JG;                    // J: matrices per pass (7)
   static constexpr int XAUX = JMPP * 81;
-  // FP64 order 2: D2 and sc1 reach the J lanes from their matrix's lane by shuffles (no aux region: the tile stays
-  // under 20 KB, eight wavefronts per CU); else staged in LDS after the J tile
-  static constexpr bool AUXR = ORDER == 2 && sizeof(OT) == 8;
+  // FP64: D2 and sc1 reach the J lanes from their matrix's lane by shuffles (no aux region: order 2 stays under
+  // 20 KB, eight wavefronts per CU; order 1 at 14 KB, eleven); FP32: staged in LDS after the J tile
+  static constexpr bool AUXR = sizeof(OT) == 8;
   static constexpr int HT = sizeof(OT) == 4 ? 1458 : 729;  // H passes: both matrices (FP32), else one at a time
   static constexpr int XJ = XAUX + (AUXR ? 0 : 448);
-  static constexpr int XS = ORDER == 0 ? 576 : (ORDER == 1 || XJ > HT ? XJ : HT);
+  static constexpr int XS0 = ORDER == 0 || XJ < 576 ? 576 : XJ;   // (F of the 64 matrices: 576)
+  static constexpr int XS = ORDER >= 2 && HT > XS0 ? HT : XS0;
   static constexpr int MAIN = ORDER == 0 ? 0 : 64 * MS;
   static constexpr int TOT = MAIN + XS;
   static constexpr int BYTES = (TOT * (int)sizeof(OT) > WAVE * 9 * 8) ? TOT * (int)sizeof(OT) : WAVE * 9 * 8;
