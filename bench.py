@@ -349,6 +349,33 @@ def main():
             res[fname(k, al)] = e
         return res
 
+    def hencky_leg(name, fn, dt, steps, warmup):
+        """mf_hip_hencky(_mixed) over the n resident results L, J, H of smf log at order 2 (the step after the
+        evaluation in a Newton iteration on psi = 1/2 |log X|^2): reads L, J, H, writes psi, g, K"""
+        fill = lib.smf_hip_eval_matrix_mixed if dt == torch.float32 else lib.smf_hip_eval_matrix
+        assert fill(1, 0.0, n, dA.data_ptr(), Fb.data_ptr(), Jb.data_ptr(), Hb.data_ptr(), sh) == 0
+        psi = torch.empty(n, dtype=dt, device=dev)
+        g = torch.empty(n * 9, dtype=dt, device=dev)
+        K = torch.empty(n * 81, dtype=dt, device=dev)
+        call = lambda: fn(n, Fb.data_ptr(), Jb.data_ptr(), Hb.data_ptr(), psi.data_ptr(), g.data_ptr(), K.data_ptr(),
+                          sh)
+        for _ in range(warmup):
+            assert call() == 0
+        torch.cuda.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record(stream)
+        for _ in range(steps):
+            assert call() == 0
+        e1.record(stream)
+        torch.cuda.synchronize()
+        ms = e0.elapsed_time(e1) / steps
+        bpm = (9 + 81 + 729 + 1 + 9 + 81) * (4 if dt == torch.float32 else 8)
+        gbs = n * bpm / (ms * 1e-3) / 1e9
+        log(f"bench: {name}: kernel avg {ms:.3f} ms, {gbs:.0f} GB/s ({gbs / HBM_PEAK_GBS:.1%} of HBM peak), "
+            f"{n / ms / 1e3:.1f} M matrices/s")
+        return {"matrices_per_s": n / (ms * 1e-3), "avg_kernel_ms": ms, "n": n, "bytes_per_matrix": bpm,
+                "achieved_GBs": gbs, "frac": gbs / HBM_PEAK_GBS}
+
     funcs_mx = FUNCS[1:]          # (exp of eigenvalues up to 1e2 is beyond the FP32 range of the mixed form)
     head = leg("smf fp64", lib.smf_hip_eval_matrix, dA, n, 8, args.steps, args.warmup)
     legs = {"smf_fp64": head}
@@ -375,6 +402,9 @@ def main():
         legs["smf_mixed"]["speedup_vs_fp64"] = t64_smf / tmx_smf
         legs["gmf_mixed"]["speedup_vs_fp64"] = legs["gmf_mixed"]["matrices_per_s"] / legs["gmf_fp64"]["matrices_per_s"]
         legs["mixed_vs_fp64_smf_plus_gmf"] = t64 / tmx
+        legs["hencky_fp64"] = hencky_leg("hencky fp64", lib.mf_hip_hencky, torch.float64, args.steps, args.warmup)
+        legs["hencky_mixed"] = hencky_leg("hencky mixed", lib.mf_hip_hencky_mixed, torch.float32, args.steps,
+                                          args.warmup)
 
     if not args.only_headline:
         del Fb, Jb, Hb

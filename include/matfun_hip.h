@@ -149,6 +149,20 @@ int mf_hip_sequence_create(const mf_hip_plan *plan, const double *A0, double *A,
 int mf_hip_sequence_launch(mf_hip_sequence *seq, void *stream);
 int mf_hip_sequence_destroy(mf_hip_sequence *seq);
 
+/* ---- Newton step of the Hencky-type potential psi(X) = 1/2 |log X|^2 (reference theory.md:190: the Newton method
+ * for this potential with the tangent J^T J + log X : H; no counterpart in the reference's API).  From L = log X,
+ * J = dL/dX and H = d2L/dX2 in the matrix layout -- as gmf_hip_eval(1, ...) gives them for any real X with positive
+ * real eigenvalues, smf_hip_eval_matrix(1, ...) for symmetric positive definite X (the *_mixed forms for the FP32
+ * entry point) -- per matrix, summed over a = 0..8:
+ *   psi = 1/2 L_a L_a (n),   g_c = L_a J_ac (n,3,3) = dpsi/dX,   K_ce = J_ac J_ae + L_a H_ace (n,3,3,3,3) = dg/dX.
+ * psi, g, K may each be NULL (not computed); L is needed for any output, J for g or K, H for K (else -1).  Sums in
+ * FP64 for both forms.  Device pointers; asynchronous on `stream`; n == 0 returns 0 without a launch; NaN inputs
+ * give NaN outputs for their matrix only.  Returns 0, -1 invalid arguments, -2 launch error. */
+int mf_hip_hencky(int64_t n, const double *L, const double *J, const double *H, double *psi, double *g, double *K,
+                  void *stream);
+int mf_hip_hencky_mixed(int64_t n, const float *L, const float *J, const float *H, float *psi, float *g, float *K,
+                        void *stream);
+
 /* ---- tile choice of the order-2 symmetric kernels (no counterpart in the reference), per function (kind 0..3;
  * 4 = pow with per-matrix exponents), layout (matrix_layout 0 Voigt, 1 matrix) and precision (mixed 0 FP64, 1 FP32
  * outputs), for the whole process: 1 = 10 matrices per H store pass with the eigenvectors / second divided

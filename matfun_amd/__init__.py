@@ -19,8 +19,8 @@ import math
 import os
 
 __version__ = "0.1.3"
-__all__ = ["smf", "smf_packed", "expand_packed", "gmf", "Batch", "Group", "library", "library_path", "backends",
-           "KINDS"]
+__all__ = ["smf", "smf_packed", "expand_packed", "gmf", "hencky", "Batch", "Group", "library", "library_path",
+           "backends", "KINDS"]
 
 KINDS = {"exp": 0, "log": 1, "pow": 2, "sqrt": 3}
 _HERE = os.path.dirname(os.path.abspath(__file__))
@@ -50,6 +50,8 @@ def library():
         for fn in (lib.smf_hip_eval_pow, lib.smf_hip_eval_matrix_pow, lib.gmf_hip_eval_pow,
                    lib.smf_hip_eval_pow_mixed, lib.smf_hip_eval_matrix_pow_mixed, lib.gmf_hip_eval_pow_mixed):
             fn.argtypes, fn.restype = pargs, ctypes.c_int
+        for fn in (lib.mf_hip_hencky, lib.mf_hip_hencky_mixed):
+            fn.argtypes, fn.restype = [ctypes.c_int64] + [ctypes.c_void_p] * 7, ctypes.c_int
         lib.smf_hip_set_tile.argtypes = [ctypes.c_int] * 4
         lib.smf_hip_get_tile.argtypes = [ctypes.c_int] * 3
         lib.smf_hip_version.restype = ctypes.c_char_p
@@ -208,6 +210,48 @@ def gmf(kind, A, alpha=None, order=0, precision="fp64"):
     fn = ((lib.gmf_hip_eval_pow_mixed if mixed else lib.gmf_hip_eval_pow) if pa else
           (lib.gmf_hip_eval_mixed if mixed else lib.gmf_hip_eval))
     return _run(fn, kind, alpha, A, shape[:-2], ((3, 3), (3, 3, 3, 3), (3, 3, 3, 3, 3, 3)), order, dt, pa)
+
+
+def hencky(L, J, H=None):
+    """value, gradient and tangent of the Hencky-type potential psi(X) = 1/2 |log X|^2 (reference theory.md:190):
+    from L = log X (..., 3, 3), J = dL/dX (..., 3, 3, 3, 3) and optionally H = d2L/dX2 (..., 3^6), as
+    gmf("log", X, order=2) or smf("log", X 3x3, order=2) return them, per matrix
+        psi = 1/2 L:L (...),   g = J^T L (..., 3, 3) = dpsi/dX,   K = J^T J + L:H (..., 3, 3, 3, 3) = dg/dX
+    (K is None without H).  float64 inputs run mf_hip_hencky, float32 ones (precision "mixed") mf_hip_hencky_mixed;
+    sums in FP64 for both.  torch CUDA tensors stay on the device (torch's current stream); NumPy arrays are copied
+    to the GPU and the results come back as NumPy arrays."""
+    import torch
+    to_numpy = not isinstance(L, torch.Tensor) or not L.is_cuda
+    if isinstance(L, torch.Tensor):
+        dt = L.dtype
+    else:
+        import numpy as np
+        dt = torch.float32 if np.asarray(L).dtype == np.float32 else torch.float64
+    if dt not in (torch.float64, torch.float32):
+        raise ValueError(f"expected float64 or float32 inputs, got {dt}")
+    ins = [torch.as_tensor(x, dtype=dt) for x in (L, J, H) if x is not None]
+    ins = [(x if x.is_cuda else x.to("cuda")).contiguous() for x in ins]
+    lead = tuple(ins[0].shape[:-2])
+    for x, tail, name in zip(ins, ((3, 3), (3,) * 4, (3,) * 6), ("L", "J", "H")):
+        if tuple(x.shape) != lead + tail:
+            raise ValueError(f"{name} has shape {tuple(x.shape)}, expected {lead + tail}")
+    n = math.prod(lead)
+    dev = ins[0].device
+    psi = torch.empty(lead, dtype=dt, device=dev)
+    g = torch.empty(lead + (3, 3), dtype=dt, device=dev)
+    K = torch.empty(lead + (3,) * 4, dtype=dt, device=dev) if H is not None else None
+    lib = library()
+    fn = lib.mf_hip_hencky_mixed if dt == torch.float32 else lib.mf_hip_hencky
+    ptr = (lambda t: t.data_ptr() if t is not None and n else None)
+    with torch.cuda.device(dev):
+        r = fn(n, ptr(ins[0]), ptr(ins[1]), ptr(ins[2]) if H is not None else None, ptr(psi), ptr(g), ptr(K),
+               _stream_handle(torch))
+    if r != 0:
+        raise ValueError(f"{fn.__name__} returned {r}")
+    out = (psi, g, K)
+    if to_numpy:
+        out = tuple(o.cpu().numpy() if o is not None else None for o in out)
+    return out
 
 
 # ---- heterogeneous batches (include/matfun_hip.h, mf_hip_*)

@@ -104,3 +104,23 @@ class Checker:
     def gmf_terms(self, kind, alpha, order, A):
         A = np.ascontiguousarray(A, dtype=np.float64).reshape(9)
         return self.lib.oracle_gmf_terms(self._kind(kind), float(alpha), int(order), A.ctypes.data)
+
+
+def hencky(L, J, H=None):
+    """The Newton step of the Hencky-type potential psi(X) = 1/2 |log X|^2 (reference theory.md:190: the Newton method
+    for this potential with the tangent J^T J + log X : H), restated in NumPy from L = log X (n, 9), J = dL/dX (n, 81)
+    and H = d2L/dX2 (n, 729) in the matrix layout: per matrix, summed over a,
+        psi = 1/2 L_a L_a,   g_c = L_a J_ac,   K_ce = J_ac J_ae + L_a H_ace.
+    Returns (psi (n,), g (n, 9), K (n, 81) or None).  Pinned in tests/test_hencky_oracle.py against what the
+    mathematics fixes: g and K equal central differences of psi and g over X (through this oracle's gmf log), K is
+    symmetric, and diagonal X gives the closed forms g = diag(log x / x), K_(cc)(cc) = (1 - log x_c) / x_c^2."""
+    L = np.asarray(L, dtype=np.float64).reshape(-1, 9)
+    n = L.shape[0]
+    Jm = np.asarray(J, dtype=np.float64).reshape(n, 9, 9)
+    psi = 0.5 * np.einsum("na,na->n", L, L)
+    g = np.einsum("na,nac->nc", L, Jm)
+    K = None
+    if H is not None:
+        Hm = np.asarray(H, dtype=np.float64).reshape(n, 9, 9, 9)
+        K = (np.einsum("nac,nae->nce", Jm, Jm) + np.einsum("na,nace->nce", L, Hm)).reshape(n, 81)
+    return psi, g, K

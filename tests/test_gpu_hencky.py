@@ -1,0 +1,149 @@
+"""mf_hip_hencky / mf_hip_hencky_mixed -- value, gradient and tangent of psi(X) = 1/2 |log X|^2 from L = log X, J, H
+(reference theory.md:190) -- on the GPU through the C ABI, against the oracle (oracle.pyoracle.hencky, pinned by
+finite differences and closed forms in tests/test_hencky_oracle.py) on the same L, J, H, which come from this
+library's own gmf / smf log at order 2.
+Tolerances (per matrix, normwise): FP64 1e-13 (the kernel's FMA chain and NumPy's sums differ in order only);
+FP32 form 1e-6 against the oracle on the same FP32 inputs (the kernel sums in FP64 and rounds once to FP32).
+At 2^20 matrices: against a plain PyTorch FP64 reference on the device, and the symmetry of K."""
+import numpy as np
+import pytest
+
+import matfun_amd
+from oracle.pyoracle import hencky as hencky_oracle
+from tests.gpu_util import call
+from tests.parity import normwise
+
+pytestmark = pytest.mark.gpu
+
+
+def general_pos(n, seed):
+    rng = np.random.default_rng(seed)
+    return (2.0 * np.eye(3) + 0.6 * rng.standard_normal((n, 3, 3))).reshape(n, 9)
+
+
+def spd_pos(n, seed):
+    rng = np.random.default_rng(seed)
+    M = rng.standard_normal((n, 3, 3))
+    return (np.einsum("nij,nkj->nik", M, M) + 0.5 * np.eye(3)).reshape(n, 9)
+
+
+def run(L, J, H, want=(True, True, True), mixed=False):
+    """mf_hip_hencky(_mixed) on host arrays L (n,9), J (n,81), H (n,729) or None; outputs as asked (psi, g, K),
+    prefilled with 7 (never a result by accident); returns rc and the outputs (NumPy, float64)"""
+    import torch
+    lib = matfun_amd.library()
+    dt = torch.float32 if mixed else torch.float64
+    n = L.shape[0]
+    dev = lambda a: None if a is None else torch.from_numpy(np.ascontiguousarray(a)).to("cuda", dt)
+    dL, dJ, dH = dev(L), dev(J), dev(H)
+    outs = [torch.full(s, 7.0, dtype=dt, device="cuda") if w else None
+            for s, w in zip(((n,), (n, 9), (n, 81)), want)]
+    p = lambda t: None if t is None else t.data_ptr()
+    fn = lib.mf_hip_hencky_mixed if mixed else lib.mf_hip_hencky
+    rc = fn(n, p(dL), p(dJ), p(dH), *[p(o) for o in outs], torch.cuda.current_stream().cuda_stream)
+    torch.cuda.synchronize()
+    return rc, [None if o is None else o.cpu().double().numpy() for o in outs]
+
+
+def inputs(fam, n, seed, mixed=False):
+    X = general_pos(n, seed) if fam == "general" else spd_pos(n, seed)
+    rc, L, J, H = call(fam, "log", 0.0, X, 2, mixed=mixed)
+    assert rc == 0
+    return L, J, H
+
+
+@pytest.mark.parametrize("fam", ["general", "matrix"])
+@pytest.mark.parametrize("n", [1, 31, 32, 33, 257, 1000])
+def test_fp64_matches_oracle(fam, n):
+    L, J, H = inputs(fam, n, 10 + n)
+    rc, (psi, g, K) = run(L, J, H)
+    assert rc == 0
+    rpsi, rg, rK = hencky_oracle(L, J, H)
+    assert normwise(psi[:, None], rpsi[:, None]).max() <= 1e-13
+    assert normwise(g, rg).max() <= 1e-13
+    assert normwise(K, rK).max() <= 1e-13
+
+
+@pytest.mark.parametrize("fam", ["general", "matrix"])
+def test_mixed_matches_oracle(fam):
+    n = 515
+    L, J, H = inputs(fam, n, 3, mixed=True)               # FP32 values (returned as float64 copies)
+    rc, (psi, g, K) = run(L, J, H, mixed=True)
+    assert rc == 0
+    rpsi, rg, rK = hencky_oracle(L, J, H)
+    assert normwise(psi[:, None], rpsi[:, None]).max() <= 1e-6
+    assert normwise(g, rg).max() <= 1e-6
+    assert normwise(K, rK).max() <= 1e-6
+
+
+def test_optional_outputs_equal_the_full_call():
+    n = 100
+    L, J, H = inputs("general", n, 5)
+    rc, (psi, g, K) = run(L, J, H)
+    assert rc == 0
+    rc1, (p1, g1, k1) = run(L, None, None, want=(True, False, False))     # psi from L alone
+    rc2, (p2, g2, k2) = run(L, J, None, want=(False, True, False))        # g without H
+    rc3, (p3, g3, k3) = run(L, J, H, want=(False, False, True))
+    assert rc1 == rc2 == rc3 == 0
+    same = lambda a, b: np.array_equal(a, b, equal_nan=True)      # (a matrix outside the log's domain: NaN)
+    assert same(p1, psi) and same(g2, g) and same(k3, K)
+    assert g1 is None and k1 is None and p2 is None and k2 is None and p3 is None and g3 is None
+
+
+def test_nan_matrix_only_affects_itself():
+    n = 70
+    L, J, H = inputs("general", n, 6)
+    L[33, 4] = np.nan
+    rc, (psi, g, K) = run(L, J, H)
+    assert rc == 0
+    rpsi, rg, rK = hencky_oracle(L, J, H)
+    assert np.isnan(psi[33]) and np.isnan(g[33]).all() and np.isnan(K[33]).all()
+    keep = np.arange(n) != 33
+    assert not np.isnan(K[keep]).any()
+    assert normwise(K[keep], rK[keep]).max() <= 1e-13
+
+
+def test_invalid_arguments():
+    L, J, H = inputs("general", 4, 7)
+    assert run(L, J, None, want=(False, False, True))[0] == -1              # K without H
+    assert run(L, None, None, want=(False, True, False))[0] == -1           # g without J
+
+
+def test_host_api_matches_c_abi():
+    import torch
+    n = 64
+    L, J, H = inputs("general", n, 8)
+    rc, (psi, g, K) = run(L, J, H)
+    dL = torch.from_numpy(L.reshape(n, 3, 3)).cuda()
+    dJ = torch.from_numpy(J.reshape(n, 3, 3, 3, 3)).cuda()
+    dH = torch.from_numpy(H.reshape((n,) + (3,) * 6)).cuda()
+    tpsi, tg, tK = matfun_amd.hencky(dL, dJ, dH)
+    assert tg.shape == (n, 3, 3) and tK.shape == (n, 3, 3, 3, 3) and tpsi.shape == (n,)
+    same = lambda a, b: np.array_equal(a, b, equal_nan=True)
+    assert same(tpsi.cpu().numpy(), psi) and same(tg.cpu().numpy().reshape(n, 9), g)
+    assert same(tK.cpu().numpy().reshape(n, 81), K)
+    npsi, ng, nK = matfun_amd.hencky(L.reshape(n, 3, 3), J.reshape(n, 3, 3, 3, 3))     # NumPy in, no H
+    assert isinstance(ng, np.ndarray) and nK is None and same(ng.reshape(n, 9), g)
+
+
+def test_large_batch_against_torch_fp64():
+    import torch
+    n = 1 << 20
+    rng = torch.Generator(device="cuda").manual_seed(11)
+    X = (2.0 * torch.eye(3, dtype=torch.float64, device="cuda") +
+         0.6 * torch.randn((n, 3, 3), dtype=torch.float64, device="cuda", generator=rng)).contiguous()
+    L, J, H = matfun_amd.gmf("log", X, order=2)
+    psi, g, K = matfun_amd.hencky(L, J, H)
+    torch.cuda.synchronize()
+    Lf, Jm = L.reshape(n, 1, 9), J.reshape(n, 9, 9)
+    g_ref = torch.bmm(Lf, Jm).reshape(n, 9)
+    K_ref = torch.bmm(Jm.transpose(1, 2).contiguous(), Jm) + torch.bmm(Lf, H.reshape(n, 9, 81)).reshape(n, 9, 9)
+    psi_ref = 0.5 * (Lf.reshape(n, 9) ** 2).sum(dim=1)
+    ok = ~torch.isnan(K_ref.reshape(n, -1)).any(dim=1)
+    assert ok.float().mean().item() > 0.99
+    def rel(a, b):
+        a, b = a.reshape(n, -1)[ok], b.reshape(n, -1)[ok]
+        return ((a - b).norm(dim=1) / b.norm(dim=1)).max().item()
+    assert rel(g, g_ref) <= 1e-13 and rel(K, K_ref) <= 1e-13 and rel(psi[:, None], psi_ref[:, None]) <= 1e-13
+    Km = K.reshape(n, 9, 9)
+    assert rel(Km, Km.transpose(1, 2).contiguous()) <= 1e-12

@@ -77,3 +77,17 @@ def test_per_matrix_exponent_entry_points_reject_before_device_work():
         assert f(None, 0, None, None, None, None, None) == 0               # n == 0: nothing to do
     assert lib.smf_hip_eval_pow(dummy, 2 ** 40, dummy, dummy, None, None, None) == -2
     assert lib.gmf_hip_eval_pow(dummy, (2 ** 31 - 1) * 64 + 1, dummy, dummy, None, None, None) == -2
+
+
+@pytest.mark.skipif(not os.path.exists(LIB), reason="HIP library not built (python -m matfun_amd.build)")
+def test_hencky_entry_points_reject_before_device_work():
+    lib = ctypes.CDLL(LIB)
+    dummy = ctypes.c_void_p(0x1000)
+    for f in (lib.mf_hip_hencky, lib.mf_hip_hencky_mixed):
+        f.argtypes, f.restype = [ctypes.c_int64] + [ctypes.c_void_p] * 7, ctypes.c_int
+        assert f(-1, dummy, dummy, dummy, dummy, dummy, dummy, None) == -1      # n < 0
+        assert f(10, dummy, dummy, None, None, None, dummy, None) == -1         # K without H
+        assert f(10, dummy, None, None, None, dummy, None, None) == -1          # g without J
+        assert f(10, None, dummy, dummy, dummy, None, None, None) == -1         # no L
+        assert f(0, None, None, None, None, None, None, None) == 0              # n == 0: nothing to do
+        assert f(10, dummy, dummy, dummy, None, None, None, None) == 0          # no output asked for
