@@ -256,6 +256,7 @@ def main():
     ap.add_argument("--gmf-n", type=int, default=1 << 20)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--only-headline", action="store_true")
+    ap.add_argument("--only-hencky", action="store_true", help="the headline and the Hencky legs only")
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--mixed-n", type=int, default=1 << 24)
     ap.add_argument("--mixed-gmf-n", type=int, default=1 << 22)
@@ -379,7 +380,7 @@ def main():
     funcs_mx = FUNCS[1:]          # (exp of eigenvalues up to 1e2 is beyond the FP32 range of the mixed form)
     head = leg("smf fp64", lib.smf_hip_eval_matrix, dA, n, 8, args.steps, args.warmup)
     legs = {"smf_fp64": head}
-    if not args.only_headline:
+    if not (args.only_headline or args.only_hencky):
         legs["smf_mixed"] = leg("smf mixed", lib.smf_hip_eval_matrix_mixed, dA, n, 4, args.steps, args.warmup,
                                 funcs_mx)
         # packed symmetric outputs (F 6, J 21, H 126 per matrix) of the same matrices as Voigt vectors
@@ -402,11 +403,12 @@ def main():
         legs["smf_mixed"]["speedup_vs_fp64"] = t64_smf / tmx_smf
         legs["gmf_mixed"]["speedup_vs_fp64"] = legs["gmf_mixed"]["matrices_per_s"] / legs["gmf_fp64"]["matrices_per_s"]
         legs["mixed_vs_fp64_smf_plus_gmf"] = t64 / tmx
+
+    if not args.only_headline:
         legs["hencky_fp64"] = hencky_leg("hencky fp64", lib.mf_hip_hencky, torch.float64, args.steps, args.warmup)
         legs["hencky_mixed"] = hencky_leg("hencky mixed", lib.mf_hip_hencky_mixed, torch.float32, args.steps,
                                           args.warmup)
-
-    if not args.only_headline:
+    if not (args.only_headline or args.only_hencky):
         del Fb, Jb, Hb
         torch.cuda.empty_cache()
         legs_c = mixed_vs_fp64(torch, lib, W, dev, stream, args)
@@ -436,7 +438,7 @@ def main():
                         "frac": head["frac"], "traffic": profiled_traffic(n), "kernel": "mf::smf_kernel<KIND,true,2,double>",
                         "avg_kernel_ms": head["avg_kernel_ms"], "measured_copy_GBs": copy_gbs},
            "legs": legs, "tuned_tiles": tiles}
-    if not args.only_headline:
+    if not (args.only_headline or args.only_hencky):
         out["mixed_vs_fp64"] = legs_c
         out["hetero_batch"] = legs_b
     if not args.no_cpu:

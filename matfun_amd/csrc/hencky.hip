@@ -14,10 +14,11 @@
 
 #include "../../include/matfun_hip.h"
 
+namespace mf {
 namespace {
-
 constexpr int MB = 32;    // matrices per workgroup
 constexpr int NT = 256;   // work-items per workgroup
+}  // namespace
 
 template <class T>
 __global__ void __launch_bounds__(NT) hencky_kernel(int64_t n, const T* __restrict__ L, const T* __restrict__ J,
@@ -72,7 +73,7 @@ __global__ void __launch_bounds__(NT) hencky_kernel(int64_t n, const T* __restri
 
 // arguments as the entry points take them; batches above 2^30 matrices go in several launches on the stream
 template <class T>
-int hencky(int64_t n, const T* L, const T* J, const T* H, T* psi, T* g, T* K, void* stream) {
+int hencky_launch(int64_t n, const T* L, const T* J, const T* H, T* psi, T* g, T* K, void* stream) {
   if (n < 0 || ((psi || g || K) && n > 0 && !L) || ((g || K) && n > 0 && !J) || (K && n > 0 && !H)) return -1;
   if (n == 0 || (!psi && !g && !K)) return 0;
   const hipStream_t st = (hipStream_t)stream;
@@ -87,14 +88,14 @@ int hencky(int64_t n, const T* L, const T* J, const T* H, T* psi, T* g, T* K, vo
   return 0;
 }
 
-}  // namespace
+}  // namespace mf
 
 extern "C" int mf_hip_hencky(int64_t n, const double* L, const double* J, const double* H, double* psi, double* g,
                              double* K, void* stream) {
-  return hencky<double>(n, L, J, H, psi, g, K, stream);
+  return mf::hencky_launch<double>(n, L, J, H, psi, g, K, stream);
 }
 
 extern "C" int mf_hip_hencky_mixed(int64_t n, const float* L, const float* J, const float* H, float* psi, float* g,
                                    float* K, void* stream) {
-  return hencky<float>(n, L, J, H, psi, g, K, stream);
+  return mf::hencky_launch<float>(n, L, J, H, psi, g, K, stream);
 }
