@@ -377,6 +377,28 @@ def main():
         return {"matrices_per_s": n / (ms * 1e-3), "avg_kernel_ms": ms, "n": n, "bytes_per_matrix": bpm,
                 "achieved_GBs": gbs, "frac": gbs / HBM_PEAK_GBS}
 
+    def fused_leg(steps, warmup):
+        """smf_hip_hencky_matrix: log X, psi, g, K of the n symmetric inputs in one kernel (no J / H in memory)"""
+        outs = [torch.empty(n * k, dtype=torch.float64, device=dev) for k in (9, 1, 9, 81)]
+        call = lambda: lib.smf_hip_hencky_matrix(n, dA.data_ptr(), *[o.data_ptr() for o in outs], sh)
+        for _ in range(warmup):
+            assert call() == 0
+        torch.cuda.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record(stream)
+        for _ in range(steps):
+            assert call() == 0
+        e1.record(stream)
+        torch.cuda.synchronize()
+        ms = e0.elapsed_time(e1) / steps
+        bpm = 72 + 8 * (9 + 1 + 9 + 81)
+        gbs = n * bpm / (ms * 1e-3) / 1e9
+        composed = n / head["per_function_matrices_per_s"]["log"] + n / legs["hencky_fp64"]["matrices_per_s"]
+        log(f"bench: hencky fused smf fp64: kernel avg {ms:.3f} ms, {n / ms / 1e3:.1f} M matrices/s, "
+            f"{composed / (ms * 1e-3):.2f}x smf log order 2 + hencky ({composed * 1e3:.3f} ms)")
+        return {"matrices_per_s": n / (ms * 1e-3), "avg_kernel_ms": ms, "n": n, "bytes_per_matrix": bpm,
+                "achieved_GBs": gbs, "frac": gbs / HBM_PEAK_GBS, "speedup_vs_eval_plus_step": composed / (ms * 1e-3)}
+
     funcs_mx = FUNCS[1:]          # (exp of eigenvalues up to 1e2 is beyond the FP32 range of the mixed form)
     head = leg("smf fp64", lib.smf_hip_eval_matrix, dA, n, 8, args.steps, args.warmup)
     legs = {"smf_fp64": head}
@@ -408,6 +430,7 @@ def main():
         legs["hencky_fp64"] = hencky_leg("hencky fp64", lib.mf_hip_hencky, torch.float64, args.steps, args.warmup)
         legs["hencky_mixed"] = hencky_leg("hencky mixed", lib.mf_hip_hencky_mixed, torch.float32, args.steps,
                                           args.warmup)
+        legs["hencky_fused_smf_fp64"] = fused_leg(args.steps, args.warmup)
     if not (args.only_headline or args.only_hencky):
         del Fb, Jb, Hb
         torch.cuda.empty_cache()

@@ -9,10 +9,23 @@
 // loads, then work-item i of the block's 32 x 81 outputs K_(m, c, e) reads H_(m, a, c, e) for a = 0..8 straight
 // from HBM (consecutive i: consecutive addresses; every H value is read once) and writes K in the same flat order.
 // Sums in FP64 (FMA), also for the FP32 form (FP32 in and out, as the *_mixed evaluations give them).
+//
+// smf_hencky_kernel: the same step fused with the evaluation for symmetric X (matrix layout, FP64): one lane per
+// matrix computes log X, the 21 distinct J values and the 10 second divided differences as the smf kernels do
+// (SmfLane), then contracts in the Voigt basis without forming J or H in the API layout: with m_r = 1 (r < 3) or 2
+// (the number of matrix entries of Voigt index r), w_p = dv/dX = 1 or 1/2 and Hv the rows of the Voigt second
+// derivatives (SmfHess::h_row, one row at a time),
+//   psi = 1/2 sum_r m_r L_r^2,   g_p = sum_r m_r L_r Js_rp,
+//   K_pq = sum_r m_r (Js_rp Js_rq + L_r Hv_r,pq w_p w_q)
+// which are the sums over a = 0..8 above, two equal terms at a time (J_ac = Js[v(a), v(c)], H_ace = Hv[v(a)][v(c),
+// v(e)] w_c w_e in the matrix layout).  Outputs (log X, psi, g, K: 91 + 9 values, 728 B + 72 B per matrix instead
+// of 6.6 KB of J and H written and read again) staged per matrix in LDS and written as contiguous blocks.
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
 #include "../../include/matfun_hip.h"
+#include "smf_device.hpp"
+#include "tile_rows.hpp"
 
 namespace mf {
 namespace {
@@ -88,7 +101,110 @@ int hencky_launch(int64_t n, const T* L, const T* J, const T* H, T* psi, T* g, T
   return 0;
 }
 
+constexpr int SK = 35;    // fused kernel: staged values per matrix (K 21, g 6, psi, log X 6; odd stride)
+
+__global__ void __launch_bounds__(WAVE) __attribute__((amdgpu_waves_per_eu(2)))
+smf_hencky_kernel(int64_t n, const double* __restrict__ x, double* __restrict__ Lo, double* __restrict__ psi,
+                  double* __restrict__ g, double* __restrict__ K, SmfCoef c2) {
+  __shared__ __attribute__((aligned(16))) double st[WAVE * SK];   // (the 64 x 9 input block first)
+  const int64_t base = (int64_t)blockIdx.x * WAVE;
+  const int nm = (int)(n - base < WAVE ? n - base : WAVE);
+  const int lane = threadIdx.x;
+  const RunRows rows{base};
+  load_in<9>(st, x, rows, nm, lane);
+  __syncthreads();
+  double v[6];
+  {
+    double a[9];
+#pragma unroll
+    for (int k = 0; k < 9; k++) a[k] = lane < nm ? st[9 * lane + k] : (k % 4 == 0 ? 1.0 : 0.0);
+    matrix_to_voigt(a, v);                       // (NaN for a non-symmetric matrix)
+  }
+  __syncthreads();
+  SmfLane<LOG, double> s;
+  double Lv[6], Js[21];
+  s.eigen(v, 0.0, Lv);
+  s.jacobian(0.0, Js);
+  s.template second_dd<false, SMF_NTERMS>(0.0, c2.c);
+  double kv[21], gv[6], ps = 0.0;
+#pragma unroll
+  for (int r = 0; r < 6; r++) ps = fma(r < 3 ? Lv[r] : 2.0 * Lv[r], Lv[r], ps);
+#pragma unroll
+  for (int p = 0; p < 6; p++) {
+    double acc = 0.0;
+#pragma unroll
+    for (int r = 0; r < 6; r++) acc = fma(r < 3 ? Lv[r] : 2.0 * Lv[r], Js[tri6(r, p)], acc);
+    gv[p] = acc;
+  }
+#pragma unroll
+  for (int p = 0; p < 6; p++)
+#pragma unroll
+    for (int q = p; q < 6; q++) {
+      double acc = 0.0;
+#pragma unroll
+      for (int r = 0; r < 6; r++) acc = fma(r < 3 ? Js[tri6(r, p)] : 2.0 * Js[tri6(r, p)], Js[tri6(r, q)], acc);
+      kv[tri6(p, q)] = acc;
+    }
+  {
+    double V9[9];
+#pragma unroll
+    for (int k = 0; k < 9; k++) V9[k] = s.V[k / 3][k % 3];
+    const SmfHess<double> hs(V9, s.D2);
+#pragma unroll
+    for (int r = 0; r < 6; r++) {
+      double h[21];
+      hs.h_row(r, h);
+      const double c = r < 3 ? Lv[r] : 2.0 * Lv[r];
+#pragma unroll
+      for (int p = 0; p < 6; p++)
+#pragma unroll
+        for (int q = p; q < 6; q++) {
+          const double w = (p < 3 ? 1.0 : 0.5) * (q < 3 ? 1.0 : 0.5);
+          kv[tri6(p, q)] = fma(c, h[tri6(p, q)] * w, kv[tri6(p, q)]);
+        }
+    }
+  }
+  double* sm = st + SK * lane;
+#pragma unroll
+  for (int k = 0; k < 21; k++) sm[k] = kv[k];
+#pragma unroll
+  for (int k = 0; k < 6; k++) sm[21 + k] = gv[k];
+  sm[27] = 0.5 * ps;
+#pragma unroll
+  for (int k = 0; k < 6; k++) sm[28 + k] = Lv[k];
+  __syncthreads();
+  if (Lo) copy_out<9, 1, R_IN>(Lo, rows, 9 * nm, lane, [&](int j, int e) { return st[SK * j + 28 + voigt_of_m(e)]; });
+  if (psi) copy_out<1, 1, R_IN>(psi, rows, nm, lane, [&](int j, int) { return st[SK * j + 27]; });
+  if (g) copy_out<9, 1, R_IN>(g, rows, 9 * nm, lane, [&](int j, int e) { return st[SK * j + 21 + voigt_of_m(e)]; });
+  if (K)
+    copy_out<81, 1, R_IN>(K, rows, 81 * nm, lane, [&](int j, int e) {
+      return st[SK * j + tri6(voigt_of_m(e / 9), voigt_of_m(e % 9))];
+    });
+}
+
+int smf_hencky_launch(int64_t n, const double* A, double* Lo, double* psi, double* g, double* K, void* stream) {
+  if (n < 0 || ((Lo || psi || g || K) && n > 0 && !A)) return -1;
+  if (n == 0 || (!Lo && !psi && !g && !K)) return 0;
+  SmfCoef c2;
+  smf_coefs(LOG, 0.0, c2);
+  const hipStream_t st = (hipStream_t)stream;
+  constexpr int64_t CHUNK = (int64_t)1 << 30;
+  for (int64_t b = 0; b < n; b += CHUNK) {
+    const int64_t c = n - b < CHUNK ? n - b : CHUNK;
+    smf_hencky_kernel<<<dim3((unsigned)((c + WAVE - 1) / WAVE)), dim3(WAVE), 0, st>>>(
+        c, A + b * 9, Lo ? Lo + b * 9 : nullptr, psi ? psi + b : nullptr, g ? g + b * 9 : nullptr,
+        K ? K + b * 81 : nullptr, c2);
+    if (hipGetLastError() != hipSuccess) return -2;
+  }
+  return 0;
+}
+
 }  // namespace mf
+
+extern "C" int smf_hip_hencky_matrix(int64_t n, const double* A, double* L, double* psi, double* g, double* K,
+                                     void* stream) {
+  return mf::smf_hencky_launch(n, A, L, psi, g, K, stream);
+}
 
 extern "C" int mf_hip_hencky(int64_t n, const double* L, const double* J, const double* H, double* psi, double* g,
                              double* K, void* stream) {

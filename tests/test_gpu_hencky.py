@@ -147,3 +147,73 @@ def test_large_batch_against_torch_fp64():
     assert rel(g, g_ref) <= 1e-13 and rel(K, K_ref) <= 1e-13 and rel(psi[:, None], psi_ref[:, None]) <= 1e-13
     Km = K.reshape(n, 9, 9)
     assert rel(Km, Km.transpose(1, 2).contiguous()) <= 1e-12
+
+
+# ---- the fused form for symmetric X: smf_hip_hencky_matrix against smf_hip_eval_matrix(log) + mf_hip_hencky
+def fused(X, want=(True, True, True, True)):
+    import torch
+    lib = matfun_amd.library()
+    n = X.shape[0]
+    dX = torch.from_numpy(np.ascontiguousarray(X, dtype=np.float64)).cuda()
+    outs = [torch.full(s, 7.0, dtype=torch.float64, device="cuda") if w else None
+            for s, w in zip(((n, 9), (n,), (n, 9), (n, 81)), want)]
+    p = lambda t: None if t is None else t.data_ptr()
+    rc = lib.smf_hip_hencky_matrix(n, p(dX), *[p(o) for o in outs], torch.cuda.current_stream().cuda_stream)
+    torch.cuda.synchronize()
+    return rc, [None if o is None else o.cpu().numpy() for o in outs]
+
+
+@pytest.mark.parametrize("n", [1, 63, 64, 65, 1000])
+def test_fused_symmetric_equals_composed(n):
+    X = spd_pos(n, 20 + n)
+    X[n // 2] = np.array([[2.0, 0.5, 0.0], [0.5, 2.0, 0.0], [0.0, 0.0, 2.0]]).ravel()     # a repeated eigenvalue
+    if n > 2:
+        X[1] = np.diag([0.3, 0.3, 0.3]).ravel()                                           # a multiple of I
+    rc, (L, psi, g, K) = fused(X)
+    assert rc == 0
+    rc2, L2, J2, H2 = call("matrix", "log", 0.0, X, 2)
+    assert rc2 == 0
+    assert np.array_equal(L, L2)                           # the same eigen-decomposition and F
+    rpsi, rg, rK = hencky_oracle(L2, J2, H2)
+    assert normwise(psi[:, None], rpsi[:, None]).max() <= 1e-13
+    assert normwise(g, rg).max() <= 1e-13
+    assert normwise(K, rK).max() <= 1e-13
+    Km = K.reshape(n, 9, 9)
+    assert np.array_equal(Km, Km.transpose(0, 2, 1))       # assembled from its distinct values
+
+
+def test_fused_nan_and_optional_outputs():
+    n = 130
+    X = spd_pos(n, 31)
+    X[70, 1] += 1e-3                                        # not symmetric: NaN for that matrix only
+    rc, (L, psi, g, K) = fused(X)
+    assert rc == 0
+    assert np.isnan(K[70]).all() and np.isnan(psi[70]) and np.isnan(L[70]).all()
+    keep = np.arange(n) != 70
+    assert not np.isnan(K[keep]).any()
+    rc1, (L1, p1, g1, K1) = fused(X, want=(False, False, False, True))
+    rc2, (L2, p2, g2, K2) = fused(X, want=(False, True, True, False))
+    assert rc1 == rc2 == 0 and L1 is None and L2 is None
+    same = lambda a, b: np.array_equal(a, b, equal_nan=True)
+    assert same(K1, K) and same(p2, psi) and same(g2, g)
+    lib = matfun_amd.library()
+    assert lib.smf_hip_hencky_matrix(-1, None, None, None, None, None, None) == -1
+    assert lib.smf_hip_hencky_matrix(5, None, None, None, None, None, None) == 0     # nothing asked for
+    assert lib.smf_hip_hencky_matrix(0, None, None, None, None, None, None) == 0
+
+
+def test_fused_large_batch_against_composed():
+    import torch
+    n = 1 << 20
+    from matfun_amd import workloads as W
+    X = torch.from_numpy(W.spd(n, seed=5)).cuda()
+    L, psi, g, K = matfun_amd.smf_hencky(X.reshape(n, 3, 3))
+    L2, J2, H2 = matfun_amd.smf("log", X.reshape(n, 3, 3), order=2)
+    psi2, g2, K2 = matfun_amd.hencky(L2, J2, H2)
+    torch.cuda.synchronize()
+    assert torch.equal(L, L2)
+
+    def rel(a, b):
+        a, b = a.reshape(n, -1), b.reshape(n, -1)
+        return ((a - b).norm(dim=1) / b.norm(dim=1).clamp_min(1e-300)).max().item()
+    assert rel(g, g2) <= 1e-13 and rel(K, K2) <= 1e-13 and rel(psi[:, None], psi2[:, None]) <= 1e-13
