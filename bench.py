@@ -377,10 +377,13 @@ def main():
         return {"matrices_per_s": n / (ms * 1e-3), "avg_kernel_ms": ms, "n": n, "bytes_per_matrix": bpm,
                 "achieved_GBs": gbs, "frac": gbs / HBM_PEAK_GBS}
 
-    def fused_leg(steps, warmup):
-        """smf_hip_hencky_matrix: log X, psi, g, K of the n symmetric inputs in one kernel (no J / H in memory)"""
-        outs = [torch.empty(n * k, dtype=torch.float64, device=dev) for k in (9, 1, 9, 81)]
-        call = lambda: lib.smf_hip_hencky_matrix(n, dA.data_ptr(), *[o.data_ptr() for o in outs], sh)
+    def fused_leg(steps, warmup, gen=False):
+        """smf_hip_hencky_matrix (gmf_hip_hencky: gen) -- log X, psi, g, K of the n symmetric (ng general) inputs in
+        one kernel, no J / H in memory"""
+        m = ng if gen else n
+        outs = [torch.empty(m * k, dtype=torch.float64, device=dev) for k in (9, 1, 9, 81)]
+        fn, X = (lib.gmf_hip_hencky, dG) if gen else (lib.smf_hip_hencky_matrix, dA)
+        call = lambda: fn(m, X.data_ptr(), *[o.data_ptr() for o in outs], sh)
         for _ in range(warmup):
             assert call() == 0
         torch.cuda.synchronize()
@@ -392,12 +395,30 @@ def main():
         torch.cuda.synchronize()
         ms = e0.elapsed_time(e1) / steps
         bpm = 72 + 8 * (9 + 1 + 9 + 81)
-        gbs = n * bpm / (ms * 1e-3) / 1e9
-        composed = n / head["per_function_matrices_per_s"]["log"] + n / legs["hencky_fp64"]["matrices_per_s"]
-        log(f"bench: hencky fused smf fp64: kernel avg {ms:.3f} ms, {n / ms / 1e3:.1f} M matrices/s, "
-            f"{composed / (ms * 1e-3):.2f}x smf log order 2 + hencky ({composed * 1e3:.3f} ms)")
-        return {"matrices_per_s": n / (ms * 1e-3), "avg_kernel_ms": ms, "n": n, "bytes_per_matrix": bpm,
-                "achieved_GBs": gbs, "frac": gbs / HBM_PEAK_GBS, "speedup_vs_eval_plus_step": composed / (ms * 1e-3)}
+        gbs = m * bpm / (ms * 1e-3) / 1e9
+        res = {"matrices_per_s": m / (ms * 1e-3), "avg_kernel_ms": ms, "n": m, "bytes_per_matrix": bpm,
+               "achieved_GBs": gbs, "frac": gbs / HBM_PEAK_GBS}
+        if gen:                                    # (the evaluation alone, for reference: gmf log at order 2)
+            Jg = torch.empty(m * 81, dtype=torch.float64, device=dev)
+            Hg = torch.empty(m * 729, dtype=torch.float64, device=dev)
+            assert lib.gmf_hip_eval(1, 0.0, m, dG.data_ptr(), outs[0].data_ptr(), Jg.data_ptr(), Hg.data_ptr(),
+                                    sh) == 0
+            torch.cuda.synchronize()
+            e0.record(stream)
+            for _ in range(steps):
+                assert lib.gmf_hip_eval(1, 0.0, m, dG.data_ptr(), outs[0].data_ptr(), Jg.data_ptr(), Hg.data_ptr(),
+                                        sh) == 0
+            e1.record(stream)
+            torch.cuda.synchronize()
+            composed = e0.elapsed_time(e1) / steps * 1e-3 + m / legs["hencky_fp64"]["matrices_per_s"]
+            del Jg, Hg
+        else:
+            composed = m / head["per_function_matrices_per_s"]["log"] + m / legs["hencky_fp64"]["matrices_per_s"]
+        res["speedup_vs_eval_plus_step"] = composed / (ms * 1e-3)
+        log(f"bench: hencky fused {'gmf' if gen else 'smf'} fp64: kernel avg {ms:.3f} ms (n = {m}), "
+            f"{m / ms / 1e3:.1f} M matrices/s, {composed / (ms * 1e-3):.2f}x log order 2 + hencky "
+            f"({composed * 1e3:.3f} ms)")
+        return res
 
     funcs_mx = FUNCS[1:]          # (exp of eigenvalues up to 1e2 is beyond the FP32 range of the mixed form)
     head = leg("smf fp64", lib.smf_hip_eval_matrix, dA, n, 8, args.steps, args.warmup)
@@ -431,6 +452,7 @@ def main():
         legs["hencky_mixed"] = hencky_leg("hencky mixed", lib.mf_hip_hencky_mixed, torch.float32, args.steps,
                                           args.warmup)
         legs["hencky_fused_smf_fp64"] = fused_leg(args.steps, args.warmup)
+        legs["hencky_fused_gmf_fp64"] = fused_leg(args.steps, args.warmup, gen=True)
     if not (args.only_headline or args.only_hencky):
         del Fb, Jb, Hb
         torch.cuda.empty_cache()

@@ -168,6 +168,9 @@ int mf_hip_hencky_mixed(int64_t n, const float *L, const float *J, const float *
  * mf_hip_hencky up to the order of the sums (per matrix 1e-13 relative).  Any output may be NULL; NaN outputs for a
  * non-symmetric matrix (as smf_hip_eval_matrix).  Returns 0, -1 (n < 0, outputs without A), -2 launch error. */
 int smf_hip_hencky_matrix(int64_t n, const double *A, double *L, double *psi, double *g, double *K, void *stream);
+/* The same for general real X (FP64; log of X with no real eigenvalue <= 0, else NaN outputs for that matrix):
+ * gmf_hip_eval(1, 0, ...) followed by mf_hip_hencky in one kernel. */
+int gmf_hip_hencky(int64_t n, const double *A, double *L, double *psi, double *g, double *K, void *stream);
 
 /* ---- tile choice of the order-2 symmetric kernels (no counterpart in the reference), per function (kind 0..3;
  * 4 = pow with per-matrix exponents), layout (matrix_layout 0 Voigt, 1 matrix) and precision (mixed 0 FP64, 1 FP32

@@ -19,8 +19,8 @@ import math
 import os
 
 __version__ = "0.1.3"
-__all__ = ["smf", "smf_packed", "expand_packed", "gmf", "hencky", "smf_hencky", "Batch", "Group", "library",
-           "library_path", "backends", "KINDS"]
+__all__ = ["smf", "smf_packed", "expand_packed", "gmf", "hencky", "smf_hencky", "gmf_hencky", "Batch", "Group",
+           "library", "library_path", "backends", "KINDS"]
 
 KINDS = {"exp": 0, "log": 1, "pow": 2, "sqrt": 3}
 _HERE = os.path.dirname(os.path.abspath(__file__))
@@ -52,8 +52,8 @@ def library():
             fn.argtypes, fn.restype = pargs, ctypes.c_int
         for fn in (lib.mf_hip_hencky, lib.mf_hip_hencky_mixed):
             fn.argtypes, fn.restype = [ctypes.c_int64] + [ctypes.c_void_p] * 7, ctypes.c_int
-        lib.smf_hip_hencky_matrix.argtypes = [ctypes.c_int64] + [ctypes.c_void_p] * 6
-        lib.smf_hip_hencky_matrix.restype = ctypes.c_int
+        for fn in (lib.smf_hip_hencky_matrix, lib.gmf_hip_hencky):
+            fn.argtypes, fn.restype = [ctypes.c_int64] + [ctypes.c_void_p] * 6, ctypes.c_int
         lib.smf_hip_set_tile.argtypes = [ctypes.c_int] * 4
         lib.smf_hip_get_tile.argtypes = [ctypes.c_int] * 3
         lib.smf_hip_version.restype = ctypes.c_char_p
@@ -260,6 +260,16 @@ def smf_hencky(X):
     """log X, psi, g, K of the Hencky-type potential for symmetric X (..., 3, 3) in one fused kernel
     (smf_hip_hencky_matrix; FP64): the results of smf("log", X, order=2) followed by hencky, without J and H.
     Returns (L, psi, g, K); NaN results for a non-symmetric matrix."""
+    return _fused_hencky(library().smf_hip_hencky_matrix, X)
+
+
+def gmf_hencky(X):
+    """the same for general real X (..., 3, 3) (gmf_hip_hencky; FP64): gmf("log", X, order=2) followed by hencky in
+    one kernel; NaN results for a matrix with a real eigenvalue <= 0"""
+    return _fused_hencky(library().gmf_hip_hencky, X)
+
+
+def _fused_hencky(fn, X):
     import torch
     to_numpy = not isinstance(X, torch.Tensor) or not X.is_cuda
     A = torch.as_tensor(X, dtype=torch.float64)
@@ -269,12 +279,10 @@ def smf_hencky(X):
     lead = tuple(A.shape[:-2])
     n = math.prod(lead)
     outs = [torch.empty(lead + s, dtype=torch.float64, device=A.device) for s in ((3, 3), (), (3, 3), (3, 3, 3, 3))]
-    lib = library()
     with torch.cuda.device(A.device):
-        r = lib.smf_hip_hencky_matrix(n, A.data_ptr() if n else None, *[o.data_ptr() if n else None for o in outs],
-                                      _stream_handle(torch))
+        r = fn(n, A.data_ptr() if n else None, *[o.data_ptr() if n else None for o in outs], _stream_handle(torch))
     if r != 0:
-        raise ValueError(f"smf_hip_hencky_matrix returned {r}")
+        raise ValueError(f"{fn.__name__} returned {r}")
     if to_numpy:
         outs = [o.cpu().numpy() for o in outs]
     return tuple(outs)

@@ -20,10 +20,17 @@
 // which are the sums over a = 0..8 above, two equal terms at a time (J_ac = Js[v(a), v(c)], H_ace = Hv[v(a)][v(c),
 // v(e)] w_c w_e in the matrix layout).  Outputs (log X, psi, g, K: 91 + 9 values, 728 B + 72 B per matrix instead
 // of 6.6 KB of J and H written and read again) staged per matrix in LDS and written as contiguous blocks.
+//
+// gmf_hencky_kernel: the same for general real X (FP64): the gmf tile's phase 1 per lane (gmf_lane), then passes of
+// two matrices: their J (9 lanes each, gmf_j_lane) into an LDS tile, K = J^T J and g from it, then the H lanes
+// (27 per matrix, lane (d, e, f) holding H_(ab)(cd)(ef) for all (a, b, c): gmf_h_y / gmf_h_out) add
+// L_(ab) H_(ab)(cd)(ef) to the three entries K_(cd)(ef), c = 0..2, that no other lane touches.  The sums run over
+// a = 0..8 in the order of hencky_kernel on the same J and H values.
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
 #include "../../include/matfun_hip.h"
+#include "gmf_tile.hpp"
 #include "smf_device.hpp"
 #include "tile_rows.hpp"
 
@@ -182,6 +189,140 @@ smf_hencky_kernel(int64_t n, const double* __restrict__ x, double* __restrict__ 
     });
 }
 
+// LDS of the fused gmf kernel, in doubles: r1, r2, D3 per matrix [64][28] (as the gmf tile), log X [64][9], the X1
+// tables of the two H lane groups [2][243], the J, K and g tiles of the pass [2][81], [2][81], [2][9]
+struct GhLds {
+  static constexpr int MS = 28, FB = WAVE * MS, YB = FB + WAVE * 9, JB = YB + 2 * 243, KB = JB + 2 * 81,
+                       GB = KB + 2 * 81, TOT = GB + 2 * 9;
+  static_assert(TOT >= WAVE * 9, "the input block overlays the start");
+};
+
+__global__ void __launch_bounds__(WAVE) gmf_hencky_kernel(int64_t n, const double* __restrict__ A,
+                                                          double* __restrict__ Lo, double* __restrict__ psi,
+                                                          double* __restrict__ g, double* __restrict__ K, GmfCoef bt) {
+  __shared__ __attribute__((aligned(16))) double lds[GhLds::TOT];
+  const int64_t base = (int64_t)blockIdx.x * WAVE;
+  const int nm = (int)(n - base < WAVE ? n - base : WAVE);
+  const int lane = threadIdx.x;
+  double* mainr = lds;
+  double* fb = lds + GhLds::FB;
+  double* jt = lds + GhLds::JB;
+  double* kt = lds + GhLds::KB;
+  double* gt = lds + GhLds::GB;
+  load_in<9>(lds, A, RunRows{base}, nm, lane);
+  wave_sync();
+  double a[9];
+#pragma unroll
+  for (int k = 0; k < 9; k++) a[k] = lane < nm ? lds[9 * lane + k] : (k % 4 == 0 ? 1.0 : 0.0);
+  wave_sync();
+  GmfOut o;
+  gmf_lane<LOG, 2, false>(a, 0.0, bt.b, o, g_gmf_recip.r, GmfSeriesLocal<LOG, 2, false>{bt.b});
+  {
+    double* my = mainr + lane * GhLds::MS;
+#pragma unroll
+    for (int k = 0; k < 9; k++) { my[k] = o.r[1][k]; my[9 + k] = o.r[2][k]; }
+#pragma unroll
+    for (int i = 0; i < 3; i++)
+#pragma unroll
+      for (int j = i; j < 3; j++)
+#pragma unroll
+        for (int k = j; k < 3; k++) my[18 + sym33s(i, j, k)] = o.D3[9 * i + 3 * j + k];
+  }
+  double ps = 0.0;
+#pragma unroll
+  for (int k = 0; k < 9; k++) {
+    fb[lane * 9 + k] = o.F[k];
+    ps = fma(o.F[k], o.F[k], ps);
+  }
+  double d2r[7];                                   // D2 (6 distinct, by sym3) and sc1 of this lane's matrix
+#pragma unroll
+  for (int i = 0; i < 3; i++)
+#pragma unroll
+    for (int j = i; j < 3; j++) d2r[sym3(i, j)] = o.D2[3 * i + j];
+  d2r[6] = o.sc1;
+  const double sc2 = o.sc2;
+  wave_sync();
+  if (Lo) copy_block<2>(Lo + base * 9, fb, 9 * nm, lane);
+  if (psi && lane < nm) psi[base + lane] = 0.5 * ps;
+  const int gj = lane / 9, Lj = lane - 9 * gj;     // J: lane (gj, Lj = 3 c + d), gj < 2
+  const int gh = lane / 27, Lg = lane - 27 * gh;   // H: lane (gh, Lg = 9 d + 3 e + f), gh < 2
+  const int d = Lg / 9, e = (Lg / 3) % 3, f = Lg % 3;
+  for (int p0 = 0; p0 < nm; p0 += 2) {
+    const int cnt = nm - p0 < 2 ? nm - p0 : 2;
+    double dj[7];                                  // (from the matrix's lane; every lane takes part)
+#pragma unroll
+    for (int k = 0; k < 7; k++) dj[k] = __shfl(d2r[k], (p0 + (gj < 2 ? gj : 0)) & (WAVE - 1));
+    const double s2 = __shfl(sc2, (p0 + (gh < 2 ? gh : 0)) & (WAVE - 1));
+    if (gj < cnt) {
+      const double* s = mainr + (p0 + gj) * GhLds::MS;
+      double out[3];
+#pragma unroll
+      for (int b = 0; b < 3; b++) {
+        gmf_j_lane(s, s + 9, dj, dj[6], b, Lj / 3, Lj % 3, out);
+#pragma unroll
+        for (int k = 0; k < 3; k++) jt[gj * 81 + 27 * k + 9 * b + Lj] = out[k];
+      }
+    }
+    const double* sh = mainr + (p0 + (gh < 2 ? gh : 0)) * GhLds::MS;
+    double* ys = lds + GhLds::YB + (gh < 2 ? gh : 0) * 243;
+    double y[9], out[27];
+    if (gh < cnt) {
+      gmf_h_y(sh, sh + 9, sh + 18, d, e, f, y);
+#pragma unroll
+      for (int k = 0; k < 9; k++) ys[9 * Lg + k] = y[k];
+    }
+    wave_sync();                                   // (J tile and X1 tables staged)
+    for (int t = lane; t < 81 * cnt; t += WAVE) {  // K = J^T J of the pass
+      const int m = t / 81, ce = t - 81 * m, c = ce / 9, ee = ce - 9 * c;
+      const double* jm = jt + 81 * m;
+      double acc = 0.0;
+#pragma unroll
+      for (int q = 0; q < 9; q++) acc = fma(jm[9 * q + c], jm[9 * q + ee], acc);
+      kt[t] = acc;
+    }
+    if (lane < 9 * cnt) {                          // g = J^T L
+      const int m = lane / 9, c = lane - 9 * m;
+      double acc = 0.0;
+#pragma unroll
+      for (int q = 0; q < 9; q++) acc = fma(fb[(p0 + m) * 9 + q], jt[81 * m + 9 * q + c], acc);
+      gt[lane] = acc;
+    }
+    if (gh < cnt) gmf_h_out(sh, sh + 9, s2, d, e, f, y, ys, out);
+    wave_sync();                                   // (K tile holds J^T J)
+    if (gh < cnt) {                                // K += L : H on this lane's entries (c, Lg)
+      const double* lm = fb + (p0 + gh) * 9;
+#pragma unroll
+      for (int c = 0; c < 3; c++) {
+        double acc = kt[gh * 81 + 27 * c + Lg];
+#pragma unroll
+        for (int q = 0; q < 9; q++) acc = fma(lm[q], out[3 * q + c], acc);
+        kt[gh * 81 + 27 * c + Lg] = acc;
+      }
+    }
+    wave_sync();
+    if (K) copy_block<2>(K + (base + p0) * 81, kt, 81 * cnt, lane);
+    if (g) copy_block<2>(g + (base + p0) * 9, gt, 9 * cnt, lane);
+    wave_sync();                                   // (the tiles read before the next pass writes them)
+  }
+}
+
+int gmf_hencky_launch(int64_t n, const double* A, double* Lo, double* psi, double* g, double* K, void* stream) {
+  if (n < 0 || ((Lo || psi || g || K) && n > 0 && !A)) return -1;
+  if (n == 0 || (!Lo && !psi && !g && !K)) return 0;
+  GmfCoef bt;
+  gmf_coefs(LOG, 0.0, bt);
+  const hipStream_t st = (hipStream_t)stream;
+  constexpr int64_t CHUNK = (int64_t)1 << 30;
+  for (int64_t b = 0; b < n; b += CHUNK) {
+    const int64_t c = n - b < CHUNK ? n - b : CHUNK;
+    gmf_hencky_kernel<<<dim3((unsigned)((c + WAVE - 1) / WAVE)), dim3(WAVE), 0, st>>>(
+        c, A + b * 9, Lo ? Lo + b * 9 : nullptr, psi ? psi + b : nullptr, g ? g + b * 9 : nullptr,
+        K ? K + b * 81 : nullptr, bt);
+    if (hipGetLastError() != hipSuccess) return -2;
+  }
+  return 0;
+}
+
 int smf_hencky_launch(int64_t n, const double* A, double* Lo, double* psi, double* g, double* K, void* stream) {
   if (n < 0 || ((Lo || psi || g || K) && n > 0 && !A)) return -1;
   if (n == 0 || (!Lo && !psi && !g && !K)) return 0;
@@ -204,6 +345,11 @@ int smf_hencky_launch(int64_t n, const double* A, double* Lo, double* psi, doubl
 extern "C" int smf_hip_hencky_matrix(int64_t n, const double* A, double* L, double* psi, double* g, double* K,
                                      void* stream) {
   return mf::smf_hencky_launch(n, A, L, psi, g, K, stream);
+}
+
+extern "C" int gmf_hip_hencky(int64_t n, const double* A, double* L, double* psi, double* g, double* K,
+                              void* stream) {
+  return mf::gmf_hencky_launch(n, A, L, psi, g, K, stream);
 }
 
 extern "C" int mf_hip_hencky(int64_t n, const double* L, const double* J, const double* H, double* psi, double* g,

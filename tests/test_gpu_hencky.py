@@ -217,3 +217,72 @@ def test_fused_large_batch_against_composed():
         a, b = a.reshape(n, -1), b.reshape(n, -1)
         return ((a - b).norm(dim=1) / b.norm(dim=1).clamp_min(1e-300)).max().item()
     assert rel(g, g2) <= 1e-13 and rel(K, K2) <= 1e-13 and rel(psi[:, None], psi2[:, None]) <= 1e-13
+
+
+# ---- the fused form for general X: gmf_hip_hencky against gmf_hip_eval(log) + mf_hip_hencky
+def fused_g(X, want=(True, True, True, True)):
+    import torch
+    lib = matfun_amd.library()
+    n = X.shape[0]
+    dX = torch.from_numpy(np.ascontiguousarray(X, dtype=np.float64)).cuda()
+    outs = [torch.full(s, 7.0, dtype=torch.float64, device="cuda") if w else None
+            for s, w in zip(((n, 9), (n,), (n, 9), (n, 81)), want)]
+    p = lambda t: None if t is None else t.data_ptr()
+    rc = lib.gmf_hip_hencky(n, p(dX), *[p(o) for o in outs], torch.cuda.current_stream().cuda_stream)
+    torch.cuda.synchronize()
+    return rc, [None if o is None else o.cpu().numpy() for o in outs]
+
+
+@pytest.mark.parametrize("n", [1, 2, 3, 63, 64, 65, 1000])
+def test_fused_general_equals_composed(n):
+    X = general_pos(n, 40 + n)
+    X[n // 2] = np.array([[2.0, 1.0, 0.0], [0.0, 2.0, 0.0], [0.0, 0.0, 3.0]]).ravel()     # a Jordan block
+    if n > 2:
+        X[1] = np.array([[1.0, -2.0, 0.0], [2.0, 1.0, 0.0], [0.0, 0.0, 0.5]]).ravel()    # a complex pair
+    rc, (L, psi, g, K) = fused_g(X)
+    assert rc == 0
+    rc2, L2, J2, H2 = call("general", "log", 0.0, X, 2)
+    assert rc2 == 0
+    same = lambda a, b: np.array_equal(a, b, equal_nan=True)
+    assert same(L, L2)                                     # the same phase 1 and F
+    rc3, (p3, g3, K3) = run(L2, J2, H2)                    # the composed GPU path
+    rpsi, rg, rK = hencky_oracle(L2, J2, H2)
+    for x, r3, r in ((psi[:, None], p3[:, None], rpsi[:, None]), (g, g3, rg), (K, K3, rK)):
+        assert normwise(x, r3).max() <= 1e-13 and normwise(x, r).max() <= 1e-13
+
+
+def test_fused_general_nan_and_optional_outputs():
+    n = 130
+    X = general_pos(n, 41)
+    X[70] = np.diag([1.0, -2.0, 3.0]).ravel()             # a negative real eigenvalue: NaN for that matrix only
+    rc, (L, psi, g, K) = fused_g(X)
+    assert rc == 0
+    assert np.isnan(K[70]).all() and np.isnan(psi[70]) and np.isnan(g[70]).all()
+    rc1, (L1, p1, g1, K1) = fused_g(X, want=(False, False, False, True))
+    rc2, (L2, p2, g2, K2) = fused_g(X, want=(True, True, True, False))
+    assert rc1 == rc2 == 0 and L1 is None and K2 is None
+    same = lambda a, b: np.array_equal(a, b, equal_nan=True)
+    assert same(K1, K) and same(p2, psi) and same(g2, g) and same(L2, L)
+    lib = matfun_amd.library()
+    assert lib.gmf_hip_hencky(-1, None, None, None, None, None, None) == -1
+    assert lib.gmf_hip_hencky(0, None, None, None, None, None, None) == 0
+
+
+def test_fused_general_large_batch_against_composed():
+    import torch
+    n = 1 << 20
+    rng = torch.Generator(device="cuda").manual_seed(12)
+    X = (2.0 * torch.eye(3, dtype=torch.float64, device="cuda") +
+         0.6 * torch.randn((n, 3, 3), dtype=torch.float64, device="cuda", generator=rng)).contiguous()
+    L, psi, g, K = matfun_amd.gmf_hencky(X)
+    L2, J2, H2 = matfun_amd.gmf("log", X, order=2)
+    psi2, g2, K2 = matfun_amd.hencky(L2, J2, H2)
+    torch.cuda.synchronize()
+    ok = ~torch.isnan(K2.reshape(n, -1)).any(dim=1)
+    assert torch.equal(torch.isnan(K.reshape(n, -1)).any(dim=1), ~ok)
+    assert torch.equal(L[ok], L2[ok])
+
+    def rel(a, b):
+        a, b = a.reshape(n, -1)[ok], b.reshape(n, -1)[ok]
+        return ((a - b).norm(dim=1) / b.norm(dim=1).clamp_min(1e-300)).max().item()
+    assert rel(g, g2) <= 1e-13 and rel(K, K2) <= 1e-13 and rel(psi[:, None], psi2[:, None]) <= 1e-13
