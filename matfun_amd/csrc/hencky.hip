@@ -21,10 +21,10 @@
 // v(e)] w_c w_e in the matrix layout).  Outputs (log X, psi, g, K: 91 + 9 values, 728 B + 72 B per matrix instead
 // of 6.6 KB of J and H written and read again) staged per matrix in LDS and written as contiguous blocks.
 //
-// gmf_hencky_kernel: the same for general real X (FP64): the gmf tile's phase 1 per lane (gmf_lane), then passes of
-// two matrices: their J (9 lanes each, gmf_j_lane) into an LDS tile, K = J^T J and g from it, then the H lanes
-// (27 per matrix, lane (d, e, f) holding H_(ab)(cd)(ef) for all (a, b, c): gmf_h_y / gmf_h_out) add
-// L_(ab) H_(ab)(cd)(ef) to the three entries K_(cd)(ef), c = 0..2, that no other lane touches.  The sums run over
+// gmf_hencky_kernel: the same for general real X (FP64): the gmf tile's phase 1 per lane (gmf_lane), then groups of
+// six matrices: their J (9 lanes each, gmf_j_lane) into an LDS tile, K = J^T J and g from it, then H passes of two
+// matrices in which the H lanes (27 per matrix, lane (d, e, f) holding H_(ab)(cd)(ef) for all (a, b, c): gmf_h_y /
+// gmf_h_out) add L_(ab) H_(ab)(cd)(ef) to the three entries K_(cd)(ef), c = 0..2, that no other lane touches.  The sums run over
 // a = 0..8 in the order of hencky_kernel on the same J and H values.
 #include <hip/hip_runtime.h>
 #include <stdint.h>
@@ -189,18 +189,19 @@ smf_hencky_kernel(int64_t n, const double* __restrict__ x, double* __restrict__ 
     });
 }
 
-// LDS of the fused gmf kernel, in doubles: r1, r2, D3 per matrix [64][28] (as the gmf tile), log X [64][9], the X1
-// tables of the two H lane groups [2][243], the J, K and g tiles of the pass [2][81], [2][81], [2][9]
+// LDS of the fused gmf kernel, in doubles: r1, r2, D3 per matrix [64][28] (as the gmf tile), log X [64][9], the J tile
+// of a group of six matrices [6][81] (the X1 tables of the two H lane groups [2][243] once K = J^T J is formed), the K
+// tile of the group [6][81]
 struct GhLds {
-  static constexpr int MS = 28, FB = WAVE * MS, YB = FB + WAVE * 9, JB = YB + 2 * 243, KB = JB + 2 * 81,
-                       GB = KB + 2 * 81, TOT = GB + 2 * 9;
-  static_assert(TOT >= WAVE * 9, "the input block overlays the start");
+  static constexpr int MS = 28, GM = 6, FB = WAVE * MS, JB = FB + WAVE * 9, KB = JB + GM * 81, TOT = KB + GM * 81;
+  static_assert(TOT >= WAVE * 9 && GM * 81 >= 2 * 243, "the input block overlays the start; X1 tables in the J tile");
 };
 
 __global__ void __launch_bounds__(WAVE) gmf_hencky_kernel(int64_t n, const double* __restrict__ A,
                                                           double* __restrict__ Lo, double* __restrict__ psi,
                                                           double* __restrict__ g, double* __restrict__ K, GmfCoef bt) {
   __shared__ __attribute__((aligned(16))) double lds[GhLds::TOT];
+  constexpr int GM = GhLds::GM;
   const int64_t base = (int64_t)blockIdx.x * WAVE;
   const int nm = (int)(n - base < WAVE ? n - base : WAVE);
   const int lane = threadIdx.x;
@@ -208,7 +209,6 @@ __global__ void __launch_bounds__(WAVE) gmf_hencky_kernel(int64_t n, const doubl
   double* fb = lds + GhLds::FB;
   double* jt = lds + GhLds::JB;
   double* kt = lds + GhLds::KB;
-  double* gt = lds + GhLds::GB;
   load_in<9>(lds, A, RunRows{base}, nm, lane);
   wave_sync();
   double a[9];
@@ -244,16 +244,15 @@ __global__ void __launch_bounds__(WAVE) gmf_hencky_kernel(int64_t n, const doubl
   wave_sync();
   if (Lo) copy_block<2>(Lo + base * 9, fb, 9 * nm, lane);
   if (psi && lane < nm) psi[base + lane] = 0.5 * ps;
-  const int gj = lane / 9, Lj = lane - 9 * gj;     // J: lane (gj, Lj = 3 c + d), gj < 2
+  const int gj = lane / 9, Lj = lane - 9 * gj;     // J: lane (gj, Lj = 3 c + d), gj < 6
   const int gh = lane / 27, Lg = lane - 27 * gh;   // H: lane (gh, Lg = 9 d + 3 e + f), gh < 2
   const int d = Lg / 9, e = (Lg / 3) % 3, f = Lg % 3;
-  for (int p0 = 0; p0 < nm; p0 += 2) {
-    const int cnt = nm - p0 < 2 ? nm - p0 : 2;
+  for (int p0 = 0; p0 < nm; p0 += GM) {            // groups of six matrices: J, K = J^T J, g, then K += L : H
+    const int cg = nm - p0 < GM ? nm - p0 : GM;
     double dj[7];                                  // (from the matrix's lane; every lane takes part)
 #pragma unroll
-    for (int k = 0; k < 7; k++) dj[k] = __shfl(d2r[k], (p0 + (gj < 2 ? gj : 0)) & (WAVE - 1));
-    const double s2 = __shfl(sc2, (p0 + (gh < 2 ? gh : 0)) & (WAVE - 1));
-    if (gj < cnt) {
+    for (int k = 0; k < 7; k++) dj[k] = __shfl(d2r[k], (p0 + (gj < GM ? gj : 0)) & (WAVE - 1));
+    if (gj < cg) {
       const double* s = mainr + (p0 + gj) * GhLds::MS;
       double out[3];
 #pragma unroll
@@ -263,16 +262,8 @@ __global__ void __launch_bounds__(WAVE) gmf_hencky_kernel(int64_t n, const doubl
         for (int k = 0; k < 3; k++) jt[gj * 81 + 27 * k + 9 * b + Lj] = out[k];
       }
     }
-    const double* sh = mainr + (p0 + (gh < 2 ? gh : 0)) * GhLds::MS;
-    double* ys = lds + GhLds::YB + (gh < 2 ? gh : 0) * 243;
-    double y[9], out[27];
-    if (gh < cnt) {
-      gmf_h_y(sh, sh + 9, sh + 18, d, e, f, y);
-#pragma unroll
-      for (int k = 0; k < 9; k++) ys[9 * Lg + k] = y[k];
-    }
-    wave_sync();                                   // (J tile and X1 tables staged)
-    for (int t = lane; t < 81 * cnt; t += WAVE) {  // K = J^T J of the pass
+    wave_sync();                                   // (J tile staged)
+    for (int t = lane; t < 81 * cg; t += WAVE) {   // K = J^T J of the group
       const int m = t / 81, ce = t - 81 * m, c = ce / 9, ee = ce - 9 * c;
       const double* jm = jt + 81 * m;
       double acc = 0.0;
@@ -280,29 +271,42 @@ __global__ void __launch_bounds__(WAVE) gmf_hencky_kernel(int64_t n, const doubl
       for (int q = 0; q < 9; q++) acc = fma(jm[9 * q + c], jm[9 * q + ee], acc);
       kt[t] = acc;
     }
-    if (lane < 9 * cnt) {                          // g = J^T L
+    if (g && lane < 9 * cg) {                      // g = J^T L, stored at once (consecutive lanes, consecutive g)
       const int m = lane / 9, c = lane - 9 * m;
       double acc = 0.0;
 #pragma unroll
       for (int q = 0; q < 9; q++) acc = fma(fb[(p0 + m) * 9 + q], jt[81 * m + 9 * q + c], acc);
-      gt[lane] = acc;
+      g[(base + p0) * 9 + lane] = acc;
     }
-    if (gh < cnt) gmf_h_out(sh, sh + 9, s2, d, e, f, y, ys, out);
-    wave_sync();                                   // (K tile holds J^T J)
-    if (gh < cnt) {                                // K += L : H on this lane's entries (c, Lg)
-      const double* lm = fb + (p0 + gh) * 9;
+    wave_sync();                                   // (J tile read: it takes the X1 tables)
+    for (int h0 = 0; h0 < cg; h0 += 2) {
+      const int cnt = cg - h0 < 2 ? cg - h0 : 2;
+      const int mh = h0 + (gh < 2 ? gh : 0);       // the lane group's matrix in the group
+      const double s2 = __shfl(sc2, (p0 + mh) & (WAVE - 1));
+      const double* sh = mainr + (p0 + mh) * GhLds::MS;
+      double* ys = jt + (gh < 2 ? gh : 0) * 243;
+      double y[9], out[27];
+      if (gh < cnt) {
+        gmf_h_y(sh, sh + 9, sh + 18, d, e, f, y);
 #pragma unroll
-      for (int c = 0; c < 3; c++) {
-        double acc = kt[gh * 81 + 27 * c + Lg];
-#pragma unroll
-        for (int q = 0; q < 9; q++) acc = fma(lm[q], out[3 * q + c], acc);
-        kt[gh * 81 + 27 * c + Lg] = acc;
+        for (int k = 0; k < 9; k++) ys[9 * Lg + k] = y[k];
       }
+      wave_sync();
+      if (gh < cnt) {
+        gmf_h_out(sh, sh + 9, s2, d, e, f, y, ys, out);
+        const double* lm = fb + (p0 + mh) * 9;     // K += L : H on this lane's entries (c, Lg)
+#pragma unroll
+        for (int c = 0; c < 3; c++) {
+          double acc = kt[mh * 81 + 27 * c + Lg];
+#pragma unroll
+          for (int q = 0; q < 9; q++) acc = fma(lm[q], out[3 * q + c], acc);
+          kt[mh * 81 + 27 * c + Lg] = acc;
+        }
+      }
+      wave_sync();                                 // (X1 tables read before the next pass writes them)
     }
-    wave_sync();
-    if (K) copy_block<2>(K + (base + p0) * 81, kt, 81 * cnt, lane);
-    if (g) copy_block<2>(g + (base + p0) * 9, gt, 9 * cnt, lane);
-    wave_sync();                                   // (the tiles read before the next pass writes them)
+    if (K) copy_block<2>(K + (base + p0) * 81, kt, 81 * cg, lane);
+    wave_sync();                                   // (K tile read before the next group writes it)
   }
 }
 
