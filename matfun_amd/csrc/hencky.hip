@@ -12,26 +12,21 @@
 //
 // smf_hencky_kernel: the same step fused with the evaluation for symmetric X (matrix layout, FP64): one lane per
 // matrix computes log X, the 21 distinct J values and the 10 second divided differences as the smf kernels do
-// (SmfLane), then contracts in the Voigt basis without forming J or H in the API layout: with m_r = 1 (r < 3) or 2
-// (the number of matrix entries of Voigt index r), w_p = dv/dX = 1 or 1/2 and Hv the rows of the Voigt second
-// derivatives (SmfHess::h_row, one row at a time),
-//   psi = 1/2 sum_r m_r L_r^2,   g_p = sum_r m_r L_r Js_rp,
-//   K_pq = sum_r m_r (Js_rp Js_rq + L_r Hv_r,pq w_p w_q)
-// which are the sums over a = 0..8 above, two equal terms at a time (J_ac = Js[v(a), v(c)], H_ace = Hv[v(a)][v(c),
-// v(e)] w_c w_e in the matrix layout).  Outputs (log X, psi, g, K: 91 + 9 values, 728 B + 72 B per matrix instead
+// (SmfLane), then contracts in the Voigt basis without forming J or H in the API layout (smf_hencky_lane,
+// hencky_device.hpp).  Outputs (log X, psi, g, K: 91 + 9 values, 728 B + 72 B per matrix instead
 // of 6.6 KB of J and H written and read again) staged per matrix in LDS and written as contiguous blocks.
 //
 // gmf_hencky_kernel: the same for general real X (FP64): the gmf tile's phase 1 per lane (gmf_lane), then groups of
 // six matrices: their J (9 lanes each, gmf_j_lane) into an LDS tile, K = J^T J and g from it, then H passes of two
 // matrices in which the H lanes (27 per matrix, lane (d, e, f) holding H_(ab)(cd)(ef) for all (a, b, c): gmf_h_y /
-// gmf_h_out) add L_(ab) H_(ab)(cd)(ef) to the three entries K_(cd)(ef), c = 0..2, that no other lane touches.  The sums run over
-// a = 0..8 in the order of hencky_kernel on the same J and H values.
+// gmf_h_out) add L_(ab) H_(ab)(cd)(ef) to the three entries K_(cd)(ef), c = 0..2, that no other lane touches.  The
+// sums run over a = 0..8 in the order of hencky_kernel on the same J and H values.
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
 #include "../../include/matfun_hip.h"
 #include "gmf_tile.hpp"
-#include "smf_device.hpp"
+#include "hencky_device.hpp"
 #include "tile_rows.hpp"
 
 namespace mf {
@@ -128,55 +123,14 @@ smf_hencky_kernel(int64_t n, const double* __restrict__ x, double* __restrict__ 
     matrix_to_voigt(a, v);                       // (NaN for a non-symmetric matrix)
   }
   __syncthreads();
-  SmfLane<LOG, double> s;
-  double Lv[6], Js[21];
-  s.eigen(v, 0.0, Lv);
-  s.jacobian(0.0, Js);
-  s.template second_dd<false, SMF_NTERMS>(0.0, c2.c);
-  double kv[21], gv[6], ps = 0.0;
-#pragma unroll
-  for (int r = 0; r < 6; r++) ps = fma(r < 3 ? Lv[r] : 2.0 * Lv[r], Lv[r], ps);
-#pragma unroll
-  for (int p = 0; p < 6; p++) {
-    double acc = 0.0;
-#pragma unroll
-    for (int r = 0; r < 6; r++) acc = fma(r < 3 ? Lv[r] : 2.0 * Lv[r], Js[tri6(r, p)], acc);
-    gv[p] = acc;
-  }
-#pragma unroll
-  for (int p = 0; p < 6; p++)
-#pragma unroll
-    for (int q = p; q < 6; q++) {
-      double acc = 0.0;
-#pragma unroll
-      for (int r = 0; r < 6; r++) acc = fma(r < 3 ? Js[tri6(r, p)] : 2.0 * Js[tri6(r, p)], Js[tri6(r, q)], acc);
-      kv[tri6(p, q)] = acc;
-    }
-  {
-    double V9[9];
-#pragma unroll
-    for (int k = 0; k < 9; k++) V9[k] = s.V[k / 3][k % 3];
-    const SmfHess<double> hs(V9, s.D2);
-#pragma unroll
-    for (int r = 0; r < 6; r++) {
-      double h[21];
-      hs.h_row(r, h);
-      const double c = r < 3 ? Lv[r] : 2.0 * Lv[r];
-#pragma unroll
-      for (int p = 0; p < 6; p++)
-#pragma unroll
-        for (int q = p; q < 6; q++) {
-          const double w = (p < 3 ? 1.0 : 0.5) * (q < 3 ? 1.0 : 0.5);
-          kv[tri6(p, q)] = fma(c, h[tri6(p, q)] * w, kv[tri6(p, q)]);
-        }
-    }
-  }
+  double Lv[6], gv[6], kv[21], ps;
+  smf_hencky_lane(v, c2.c, Lv, gv, kv, ps);
   double* sm = st + SK * lane;
 #pragma unroll
   for (int k = 0; k < 21; k++) sm[k] = kv[k];
 #pragma unroll
   for (int k = 0; k < 6; k++) sm[21 + k] = gv[k];
-  sm[27] = 0.5 * ps;
+  sm[27] = ps;
 #pragma unroll
   for (int k = 0; k < 6; k++) sm[28 + k] = Lv[k];
   __syncthreads();

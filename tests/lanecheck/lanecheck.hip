@@ -4,6 +4,7 @@
 #include <stdint.h>
 
 #include "gmf_device.hpp"
+#include "hencky_device.hpp"
 #include "philox.hpp"
 #include "smf_device.hpp"
 
@@ -145,6 +146,24 @@ extern "C" int lanecheck_smf_matrix_pow(const double* alv, int64_t n, const doub
 }
 extern "C" int lanecheck_gmf_pow(const double* alv, int64_t n, const double* A, double* F, double* J, double* H) {
   gmf_order<mf::POW, true>(H ? 2 : (J ? 1 : 0), 1.0, alv, n, A, F, J, H);
+  return 0;
+}
+
+// the fused Hencky step of symmetric matrices (hencky_device.hpp) in the matrix layout, as smf_hencky_kernel stores it
+extern "C" int lanecheck_smf_hencky(int64_t n, const double* A, double* L, double* psi, double* g, double* K) {
+  mf::SmfCoef c2;
+  mf::smf_coefs(mf::LOG, 0.0, c2);
+  for (int64_t m = 0; m < n; m++) {
+    double v[6], Lv[6], gv[6], kv[21], ps;
+    mf::matrix_to_voigt(A + 9 * m, v);
+    mf::smf_hencky_lane(v, c2.c, Lv, gv, kv, ps);
+    psi[m] = ps;
+    for (int e = 0; e < 9; e++) {
+      L[9 * m + e] = Lv[mf::voigt_of_m(e)];
+      g[9 * m + e] = gv[mf::voigt_of_m(e)];
+    }
+    for (int e = 0; e < 81; e++) K[81 * m + e] = kv[mf::tri6(mf::voigt_of_m(e / 9), mf::voigt_of_m(e % 9))];
+  }
   return 0;
 }
 

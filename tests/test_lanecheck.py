@@ -197,3 +197,23 @@ def test_perturbation_factors_are_normal_and_symmetric(lane):
     f2 = np.empty((n, 9))
     lane.lanecheck_perturb(12345, 4, n, sym.ctypes.data, 1.0, f2.ctypes.data)
     assert abs(np.corrcoef(f[1::2, 0], f2[1::2, 0])[0, 1]) < 0.01    # iterations draw independent streams
+
+
+def test_fused_hencky_lane_math_vs_oracle(lane):
+    """the fused symmetric Hencky step (hencky_device.hpp: contraction in the Voigt basis) against the oracle's Hencky
+    step (pinned by finite differences in tests/test_hencky_oracle.py) on the oracle's smf log of the same matrices"""
+    from oracle.pyoracle import hencky
+    f = lane.lanecheck_smf_hencky
+    f.argtypes = [ctypes.c_int64] + [ctypes.c_void_p] * 5
+    A = W.spd(1500, seed=35)
+    A[11, 5] += 0.1                                  # not symmetric: NaN results
+    n = len(A)
+    L, psi, g, K = np.empty((n, 9)), np.empty(n), np.empty((n, 9)), np.empty((n, 81))
+    f(n, np.ascontiguousarray(A).ctypes.data, L.ctypes.data, psi.ctypes.data, g.ctypes.data, K.ctypes.data)
+    _, rL, rJ, rH = Checker("oracle").smf_matrix("log", 0.0, A, 2)
+    rpsi, rg, rK = hencky(rL, rJ, rH)
+    for name, x, y in (("L", L, rL), ("psi", psi[:, None], rpsi[:, None]), ("g", g, rg), ("K", K, rK)):
+        e = normwise(x, y)
+        assert e.max() <= 1e-12, (name, e.max(), int(np.argmax(e)))
+    assert np.isnan(K[11]).all() and not np.isnan(np.delete(K, 11, axis=0)).any()
+
