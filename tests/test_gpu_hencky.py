@@ -286,3 +286,50 @@ def test_fused_general_large_batch_against_composed():
         a, b = a.reshape(n, -1)[ok], b.reshape(n, -1)[ok]
         return ((a - b).norm(dim=1) / b.norm(dim=1).clamp_min(1e-300)).max().item()
     assert rel(g, g2) <= 1e-13 and rel(K, K2) <= 1e-13 and rel(psi[:, None], psi2[:, None]) <= 1e-13
+
+
+def test_newton_on_the_gradient_converges_quadratically():
+    """Newton's method for g(X) = T (g = dpsi/dX of psi = 1/2 |log X|^2, T = g(X*)) with the fused step's tangent K:
+    quadratic convergence (reference theory.md:190), i.e. K is the derivative of g as the GPU computes both; with
+    the Gauss-Newton tangent J^T J (H dropped) the residual only shrinks by a constant factor per step.  X* = 2 I +
+    0.1 N(0, 1) (tangent condition <= 20), start X* + 1e-3 N(0, 1); measured on the 99th percentile of the per-matrix
+    relative residual over 4096 matrices (the oracle on the CPU: 2.5e-3, 3.7e-6, 8.4e-11, 7e-16; Gauss-Newton 2.5e-3,
+    1.7e-3, 1.2e-3, ...).  (Wider draws, 2 I + 0.3 N(0, 1), include tangents of condition 1e4 whose Newton basin the
+    start misses.)"""
+    import torch
+    n = 4096
+    gen = torch.Generator(device="cuda").manual_seed(13)
+    eye = torch.eye(3, dtype=torch.float64, device="cuda")
+    Xs = (2.0 * eye + 0.1 * torch.randn((n, 3, 3), dtype=torch.float64, device="cuda", generator=gen)).contiguous()
+    X0 = (Xs + 1e-3 * torch.randn((n, 3, 3), dtype=torch.float64, device="cuda", generator=gen)).contiguous()
+    T = matfun_amd.gmf_hencky(Xs)[2].reshape(n, 9)
+    ok = ~torch.isnan(T).any(dim=1)
+    assert ok.all()
+    T = T[ok]
+
+    def solve(gauss_newton, iters):
+        X, res = X0[ok].contiguous(), []
+        m = X.shape[0]
+        for _ in range(iters):
+            if gauss_newton:
+                L, J = matfun_amd.gmf("log", X, order=1)
+                Jm = J.reshape(m, 9, 9)
+                K = torch.bmm(Jm.transpose(1, 2).contiguous(), Jm)
+                g = torch.bmm(L.reshape(m, 1, 9), Jm).reshape(m, 9)
+            else:
+                _, _, g, K = matfun_amd.gmf_hencky(X)
+                K, g = K.reshape(m, 9, 9), g.reshape(m, 9)
+            r = g - T
+            rel = torch.nan_to_num(r.norm(dim=1) / T.norm(dim=1), nan=float("inf"))
+            res.append(torch.quantile(rel, 0.99).item())
+            dx = torch.linalg.solve_ex(K, r.unsqueeze(-1))[0].squeeze(-1)
+            X = (X.reshape(m, 9) - dx).reshape(m, 3, 3).contiguous()
+        return res
+
+    res = solve(False, 6)
+    assert res[-1] <= 1e-12, res
+    for a, b in zip(res, res[1:]):                 # quadratic while the error is above the rounding level
+        if a > 1e-9:
+            assert b <= 50.0 * a * a + 1e-13, res
+    gn = solve(True, 6)
+    assert not (gn[-1] <= 1e-9), (gn, res)
