@@ -168,6 +168,10 @@ int mf_hip_hencky_mixed(int64_t n, const float *L, const float *J, const float *
  * mf_hip_hencky up to the order of the sums (per matrix 1e-13 relative).  Any output may be NULL; NaN outputs for a
  * non-symmetric matrix (as smf_hip_eval_matrix).  Returns 0, -1 (n < 0, outputs without A), -2 launch error. */
 int smf_hip_hencky_matrix(int64_t n, const double *A, double *L, double *psi, double *g, double *K, void *stream);
+/* The same in Voigt coordinates v = (X11, X22, X33, X12, X13, X23) (n,6), as smf_hip_eval takes them: L = log X
+ * (n,6), psi, g_p = dpsi/dv_p (n,6), K_pq = d2psi/dv_p dv_q (n,6,6); these are the matrix-layout g, K summed over the
+ * entries of each Voigt index (g_p = m_p g_c, K_pq = m_p m_q K_ce with m = 1 diagonal, 2 off-diagonal). */
+int smf_hip_hencky(int64_t n, const double *v, double *L, double *psi, double *g, double *K, void *stream);
 /* The same for general real X (FP64; log of X with no real eigenvalue <= 0, else NaN outputs for that matrix):
  * gmf_hip_eval(1, 0, ...) followed by mf_hip_hencky in one kernel. */
 int gmf_hip_hencky(int64_t n, const double *A, double *L, double *psi, double *g, double *K, void *stream);

@@ -52,7 +52,7 @@ def library():
             fn.argtypes, fn.restype = pargs, ctypes.c_int
         for fn in (lib.mf_hip_hencky, lib.mf_hip_hencky_mixed):
             fn.argtypes, fn.restype = [ctypes.c_int64] + [ctypes.c_void_p] * 7, ctypes.c_int
-        for fn in (lib.smf_hip_hencky_matrix, lib.gmf_hip_hencky):
+        for fn in (lib.smf_hip_hencky_matrix, lib.smf_hip_hencky, lib.gmf_hip_hencky):
             fn.argtypes, fn.restype = [ctypes.c_int64] + [ctypes.c_void_p] * 6, ctypes.c_int
         lib.smf_hip_set_tile.argtypes = [ctypes.c_int] * 4
         lib.smf_hip_get_tile.argtypes = [ctypes.c_int] * 3
@@ -259,8 +259,12 @@ def hencky(L, J, H=None):
 def smf_hencky(X):
     """log X, psi, g, K of the Hencky-type potential for symmetric X (..., 3, 3) in one fused kernel
     (smf_hip_hencky_matrix; FP64): the results of smf("log", X, order=2) followed by hencky, without J and H.
-    Returns (L, psi, g, K); NaN results for a non-symmetric matrix."""
-    return _fused_hencky(library().smf_hip_hencky_matrix, X)
+    Returns (L, psi, g, K); NaN results for a non-symmetric matrix.  Voigt vectors X (..., 6) = (X11, X22, X33, X12,
+    X13, X23) run smf_hip_hencky: L (..., 6), g = dpsi/dv (..., 6), K = d2psi/dv2 (..., 6, 6)."""
+    lib = library()
+    if tuple(X.shape[-1:]) == (6,):
+        return _fused_hencky(lib.smf_hip_hencky, X, voigt=True)
+    return _fused_hencky(lib.smf_hip_hencky_matrix, X)
 
 
 def gmf_hencky(X):
@@ -269,16 +273,17 @@ def gmf_hencky(X):
     return _fused_hencky(library().gmf_hip_hencky, X)
 
 
-def _fused_hencky(fn, X):
+def _fused_hencky(fn, X, voigt=False):
     import torch
     to_numpy = not isinstance(X, torch.Tensor) or not X.is_cuda
     A = torch.as_tensor(X, dtype=torch.float64)
     A = (A if A.is_cuda else A.to("cuda")).contiguous()
-    if tuple(A.shape[-2:]) != (3, 3):
-        raise ValueError(f"expected matrices (..., 3, 3), got shape {tuple(A.shape)}")
-    lead = tuple(A.shape[:-2])
+    if not voigt and tuple(A.shape[-2:]) != (3, 3):
+        raise ValueError(f"expected matrices (..., 3, 3) or Voigt vectors (..., 6), got shape {tuple(A.shape)}")
+    lead = tuple(A.shape[:-1] if voigt else A.shape[:-2])
     n = math.prod(lead)
-    outs = [torch.empty(lead + s, dtype=torch.float64, device=A.device) for s in ((3, 3), (), (3, 3), (3, 3, 3, 3))]
+    shapes = ((6,), (), (6,), (6, 6)) if voigt else ((3, 3), (), (3, 3), (3, 3, 3, 3))
+    outs = [torch.empty(lead + s, dtype=torch.float64, device=A.device) for s in shapes]
     with torch.cuda.device(A.device):
         r = fn(n, A.data_ptr() if n else None, *[o.data_ptr() if n else None for o in outs], _stream_handle(torch))
     if r != 0:

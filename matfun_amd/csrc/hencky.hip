@@ -105,22 +105,29 @@ int hencky_launch(int64_t n, const T* L, const T* J, const T* H, T* psi, T* g, T
 
 constexpr int SK = 35;    // fused kernel: staged values per matrix (K 21, g 6, psi, log X 6; odd stride)
 
+// MAT: X and the outputs in the matrix layout (log X 9, g 9, K 81; g_c = g[v(c)], K_ce = K[tri6(v(c), v(e))]); else
+// in Voigt coordinates v (log X 6, g_p = dpsi/dv_p = m_p g[p], K_pq = d2psi/dv_p dv_q = m_p m_q K[tri6(p, q)], 36)
+template <bool MAT>
 __global__ void __launch_bounds__(WAVE) __attribute__((amdgpu_waves_per_eu(2)))
 smf_hencky_kernel(int64_t n, const double* __restrict__ x, double* __restrict__ Lo, double* __restrict__ psi,
                   double* __restrict__ g, double* __restrict__ K, SmfCoef c2) {
-  __shared__ __attribute__((aligned(16))) double st[WAVE * SK];   // (the 64 x 9 input block first)
+  constexpr int NIN = MAT ? 9 : 6;
+  __shared__ __attribute__((aligned(16))) double st[WAVE * SK];   // (the 64 x NIN input block first)
   const int64_t base = (int64_t)blockIdx.x * WAVE;
   const int nm = (int)(n - base < WAVE ? n - base : WAVE);
   const int lane = threadIdx.x;
   const RunRows rows{base};
-  load_in<9>(st, x, rows, nm, lane);
+  load_in<NIN>(st, x, rows, nm, lane);
   __syncthreads();
   double v[6];
-  {
+  if (MAT) {
     double a[9];
 #pragma unroll
     for (int k = 0; k < 9; k++) a[k] = lane < nm ? st[9 * lane + k] : (k % 4 == 0 ? 1.0 : 0.0);
     matrix_to_voigt(a, v);                       // (NaN for a non-symmetric matrix)
+  } else {
+#pragma unroll
+    for (int k = 0; k < 6; k++) v[k] = lane < nm ? st[6 * lane + k] : (k < 3 ? 1.0 : 0.0);
   }
   __syncthreads();
   double Lv[6], gv[6], kv[21], ps;
@@ -134,13 +141,25 @@ smf_hencky_kernel(int64_t n, const double* __restrict__ x, double* __restrict__ 
 #pragma unroll
   for (int k = 0; k < 6; k++) sm[28 + k] = Lv[k];
   __syncthreads();
-  if (Lo) copy_out<9, 1, R_IN>(Lo, rows, 9 * nm, lane, [&](int j, int e) { return st[SK * j + 28 + voigt_of_m(e)]; });
   if (psi) copy_out<1, 1, R_IN>(psi, rows, nm, lane, [&](int j, int) { return st[SK * j + 27]; });
-  if (g) copy_out<9, 1, R_IN>(g, rows, 9 * nm, lane, [&](int j, int e) { return st[SK * j + 21 + voigt_of_m(e)]; });
-  if (K)
-    copy_out<81, 1, R_IN>(K, rows, 81 * nm, lane, [&](int j, int e) {
-      return st[SK * j + tri6(voigt_of_m(e / 9), voigt_of_m(e % 9))];
-    });
+  if constexpr (MAT) {
+    if (Lo) copy_out<9, 1, R_IN>(Lo, rows, 9 * nm, lane, [&](int j, int e) { return st[SK * j + 28 + voigt_of_m(e)]; });
+    if (g) copy_out<9, 1, R_IN>(g, rows, 9 * nm, lane, [&](int j, int e) { return st[SK * j + 21 + voigt_of_m(e)]; });
+    if (K)
+      copy_out<81, 1, R_IN>(K, rows, 81 * nm, lane, [&](int j, int e) {
+        return st[SK * j + tri6(voigt_of_m(e / 9), voigt_of_m(e % 9))];
+      });
+  } else {                                       // (m_p = 1 or 2: exact)
+    if (Lo) copy_out<6, 1, R_IN>(Lo, rows, 6 * nm, lane, [&](int j, int e) { return st[SK * j + 28 + e]; });
+    if (g)
+      copy_out<6, 1, R_IN>(g, rows, 6 * nm, lane,
+                           [&](int j, int e) { return (e < 3 ? 1.0 : 2.0) * st[SK * j + 21 + e]; });
+    if (K)
+      copy_out<36, 1, R_IN>(K, rows, 36 * nm, lane, [&](int j, int e) {
+        const int p = e / 6, q = e - 6 * p;
+        return (p < 3 ? 1.0 : 2.0) * (q < 3 ? 1.0 : 2.0) * st[SK * j + tri6(p, q)];
+      });
+  }
 }
 
 // LDS of the fused gmf kernel, in doubles: r1, r2, D3 per matrix [64][28] (as the gmf tile), log X [64][9], the J tile
@@ -281,7 +300,8 @@ int gmf_hencky_launch(int64_t n, const double* A, double* Lo, double* psi, doubl
   return 0;
 }
 
-int smf_hencky_launch(int64_t n, const double* A, double* Lo, double* psi, double* g, double* K, void* stream) {
+int smf_hencky_launch(bool mat, int64_t n, const double* A, double* Lo, double* psi, double* g, double* K,
+                      void* stream) {
   if (n < 0 || ((Lo || psi || g || K) && n > 0 && !A)) return -1;
   if (n == 0 || (!Lo && !psi && !g && !K)) return 0;
   SmfCoef c2;
@@ -290,9 +310,12 @@ int smf_hencky_launch(int64_t n, const double* A, double* Lo, double* psi, doubl
   constexpr int64_t CHUNK = (int64_t)1 << 30;
   for (int64_t b = 0; b < n; b += CHUNK) {
     const int64_t c = n - b < CHUNK ? n - b : CHUNK;
-    smf_hencky_kernel<<<dim3((unsigned)((c + WAVE - 1) / WAVE)), dim3(WAVE), 0, st>>>(
-        c, A + b * 9, Lo ? Lo + b * 9 : nullptr, psi ? psi + b : nullptr, g ? g + b * 9 : nullptr,
-        K ? K + b * 81 : nullptr, c2);
+    const int nin = mat ? 9 : 6, nk = nin * nin;
+    const dim3 grid((unsigned)((c + WAVE - 1) / WAVE));
+    double *lo = Lo ? Lo + b * nin : nullptr, *ps = psi ? psi + b : nullptr, *gg = g ? g + b * nin : nullptr,
+           *kk = K ? K + b * nk : nullptr;
+    if (mat) smf_hencky_kernel<true><<<grid, dim3(WAVE), 0, st>>>(c, A + b * nin, lo, ps, gg, kk, c2);
+    else smf_hencky_kernel<false><<<grid, dim3(WAVE), 0, st>>>(c, A + b * nin, lo, ps, gg, kk, c2);
     if (hipGetLastError() != hipSuccess) return -2;
   }
   return 0;
@@ -302,7 +325,11 @@ int smf_hencky_launch(int64_t n, const double* A, double* Lo, double* psi, doubl
 
 extern "C" int smf_hip_hencky_matrix(int64_t n, const double* A, double* L, double* psi, double* g, double* K,
                                      void* stream) {
-  return mf::smf_hencky_launch(n, A, L, psi, g, K, stream);
+  return mf::smf_hencky_launch(true, n, A, L, psi, g, K, stream);
+}
+
+extern "C" int smf_hip_hencky(int64_t n, const double* v, double* L, double* psi, double* g, double* K, void* stream) {
+  return mf::smf_hencky_launch(false, n, v, L, psi, g, K, stream);
 }
 
 extern "C" int gmf_hip_hencky(int64_t n, const double* A, double* L, double* psi, double* g, double* K,

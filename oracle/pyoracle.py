@@ -124,3 +124,23 @@ def hencky(L, J, H=None):
         Hm = np.asarray(H, dtype=np.float64).reshape(n, 9, 9, 9)
         K = (np.einsum("nac,nae->nce", Jm, Jm) + np.einsum("na,nace->nce", L, Hm)).reshape(n, 81)
     return psi, g, K
+
+
+def hencky_voigt(F, J, H=None):
+    """The same step in Voigt coordinates v = (X11, X22, X33, X12, X13, X23) from the Voigt-layout results of smf log
+    (F (n, 6), J (n, 6, 6) = dF_i/dv_k, H (n, 6, 6, 6)): with m_i = 1 (i < 3) or 2 (the matrix entries of Voigt index i),
+        psi = 1/2 m_i F_i^2,   g_k = dpsi/dv_k = m_i F_i J_ik,   K_kl = d2psi/dv_k dv_l = m_i (J_ik J_il + F_i H_ikl).
+    Returns (psi (n,), g (n, 6), K (n, 36) or None).  Pinned in tests/test_hencky_oracle.py by central differences over
+    v through this oracle's Voigt smf log."""
+    F = np.asarray(F, dtype=np.float64).reshape(-1, 6)
+    n = F.shape[0]
+    m = np.array([1.0, 1.0, 1.0, 2.0, 2.0, 2.0])
+    Jv = np.asarray(J, dtype=np.float64).reshape(n, 6, 6)
+    psi = 0.5 * np.einsum("i,ni,ni->n", m, F, F)
+    g = np.einsum("i,ni,nik->nk", m, F, Jv)
+    K = None
+    if H is not None:
+        Hv = np.asarray(H, dtype=np.float64).reshape(n, 6, 6, 6)
+        K = (np.einsum("i,nik,nil->nkl", m, Jv, Jv) + np.einsum("i,ni,nikl->nkl", m, F, Hv)).reshape(n, 36)
+    return psi, g, K
+

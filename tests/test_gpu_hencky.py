@@ -333,3 +333,30 @@ def test_newton_on_the_gradient_converges_quadratically():
             assert b <= 50.0 * a * a + 1e-13, res
     gn = solve(True, 6)
     assert not (gn[-1] <= 1e-9), (gn, res)
+
+
+@pytest.mark.parametrize("n", [1, 63, 65, 1000])
+def test_fused_voigt_equals_matrix_form_and_oracle(n):
+    """smf_hip_hencky (Voigt coordinates): log X bitwise the Voigt smf log, psi bitwise the matrix form, g / K exactly
+    m_p / m_p m_q times the matrix form's entries (weights 1, 2: exact), and <= 1e-13 against the oracle's Voigt step
+    (oracle.pyoracle.hencky_voigt, pinned by differences over v) on the Voigt log of the same matrices"""
+    import torch
+    from oracle.pyoracle import hencky_voigt
+    A = spd_pos(n, 50 + n)
+    rep = [0, 4, 8, 1, 2, 5]
+    v = np.ascontiguousarray(A[:, rep])
+    L, psi, g, K = matfun_amd.smf_hencky(torch.from_numpy(v).cuda())
+    L, psi, g, K = (t.cpu().numpy() for t in (L, psi, g, K))
+    assert L.shape == (n, 6) and g.shape == (n, 6) and K.shape == (n, 6, 6)
+    rc, (Lm, pm, gm, Km) = fused(A)
+    assert rc == 0
+    mlt = np.array([1.0, 1.0, 1.0, 2.0, 2.0, 2.0])
+    assert np.array_equal(psi, pm)
+    assert np.array_equal(g, gm[:, rep] * mlt)
+    assert np.array_equal(K, Km.reshape(n, 9, 9)[:, rep][:, :, rep] * np.outer(mlt, mlt))
+    rc2, F2, J2, H2 = call("voigt", "log", 0.0, v, 2)
+    assert rc2 == 0 and np.array_equal(L, F2)
+    rpsi, rg, rK = hencky_voigt(F2, J2, H2)
+    assert normwise(psi[:, None], rpsi[:, None]).max() <= 1e-13
+    assert normwise(g, rg).max() <= 1e-13
+    assert normwise(K.reshape(n, 36), rK).max() <= 1e-13

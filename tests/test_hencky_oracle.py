@@ -114,3 +114,34 @@ def test_diagonal_closed_forms(ck, fam):
         for c, d in ((0, 1), (0, 2), (1, 2), (2, 0)):
             dd = (lx[:, c] - lx[:, d]) / (x[:, c] - x[:, d])
             np.testing.assert_allclose(K[:, 3 * c + d, 3 * c + d], dd * dd, rtol=1e-12)
+
+
+def test_voigt_gradient_and_tangent_are_derivatives(ck):
+    """hencky_voigt: g and K are the derivatives of psi and g over the Voigt coordinates (central differences through
+    the oracle's Voigt smf log), and they are the matrix-layout step summed over the entries of each Voigt index"""
+    from oracle.pyoracle import hencky_voigt
+    n = 24
+    A = spd_pos(n, 6)
+    v = A[:, [0, 4, 8, 1, 2, 5]]
+    rc, F, J, H = ck.smf("log", 0.0, v, 2)
+    psi, g, K = hencky_voigt(F, J, H)
+    h = 1e-5
+    for k in range(6):
+        e = np.zeros(6)
+        e[k] = h
+        _, Fp, Jp, _ = ck.smf("log", 0.0, v + e, 1)
+        _, Fm, Jm, _ = ck.smf("log", 0.0, v - e, 1)
+        pp, gp, _ = hencky_voigt(Fp, Jp)
+        pm, gm, _ = hencky_voigt(Fm, Jm)
+        assert np.max(np.abs((pp - pm) / (2 * h) - g[:, k]) / (1 + np.abs(g[:, k]))) < 1e-8
+        dg = (gp - gm) / (2 * h)
+        Kk = K.reshape(n, 6, 6)[:, :, k]
+        assert (np.linalg.norm(dg - Kk, axis=1) / np.maximum(np.linalg.norm(Kk, axis=1), 1.0)).max() < 1e-7
+    # against the matrix-layout step: g_p = m_p g_c, K_pq = m_p m_q K_ce for entries c of p, e of q
+    pm_, gm_, Km_ = psi_g(ck, "smf", A, 2)
+    rep = [0, 4, 8, 1, 2, 5]
+    mlt = np.array([1.0, 1.0, 1.0, 2.0, 2.0, 2.0])
+    np.testing.assert_allclose(psi, pm_, rtol=1e-13)
+    np.testing.assert_allclose(g, gm_[:, rep] * mlt, rtol=1e-12, atol=1e-14)
+    np.testing.assert_allclose(K.reshape(n, 6, 6), Km_.reshape(n, 9, 9)[:, rep][:, :, rep] * np.outer(mlt, mlt),
+                               rtol=1e-11, atol=1e-13)

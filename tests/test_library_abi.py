@@ -91,7 +91,7 @@ def test_hencky_entry_points_reject_before_device_work():
         assert f(10, None, dummy, dummy, dummy, None, None, None) == -1         # no L
         assert f(0, None, None, None, None, None, None, None) == 0              # n == 0: nothing to do
         assert f(10, dummy, dummy, dummy, None, None, None, None) == 0          # no output asked for
-    for f in (lib.smf_hip_hencky_matrix, lib.gmf_hip_hencky):
+    for f in (lib.smf_hip_hencky_matrix, lib.smf_hip_hencky, lib.gmf_hip_hencky):
         f.argtypes, f.restype = [ctypes.c_int64] + [ctypes.c_void_p] * 6, ctypes.c_int
         assert f(-1, dummy, dummy, dummy, dummy, dummy, None) == -1             # n < 0
         assert f(10, None, dummy, None, None, None, None) == -1                 # outputs without A
