@@ -41,12 +41,15 @@ MF_DEV void box_muller(uint32_t w1, uint32_t w2, double& z0, double& z1) {
 
 // the relative perturbation factors 1 + sigma z of the 9 entries of matrix m at a Newton iteration, z from
 // counter (m low, m high, iteration, block) under key (seed low, seed high), blocks 0..2 (four words -> four
-// normals per block; 12 drawn, the first 9 used).  sym: the entries (i, j) and (j, i) share one normal (number k of
-// the 6 of the upper triangle, row-major), so a symmetric matrix stays symmetric.
+// normals per block, numbered 4 b ..; the first 9 are used, and only the Box-Muller pairs that hold them are
+// computed).  sym: the entries (i, j) and (j, i) share one normal (number k of the 6 of the upper triangle, row-major),
+// so a symmetric matrix stays symmetric (normals 0..5: blocks 0 and 1 only).
 MF_DEV void perturb_factors(uint64_t seed, uint32_t iteration, uint64_t m, bool sym, double sigma, double* f9) {
   double z[12];
+  const int pairs = sym ? 3 : 5;                 // (normals used: 6 or 9; the pairs past them are not computed)
 #pragma unroll
   for (int b = 0; b < 3; b++) {
+    if (2 * b >= pairs) break;
     U4 c;
     c.x[0] = (uint32_t)m;
     c.x[1] = (uint32_t)(m >> 32);
@@ -54,7 +57,7 @@ MF_DEV void perturb_factors(uint64_t seed, uint32_t iteration, uint64_t m, bool 
     c.x[3] = (uint32_t)b;
     const U4 w = philox4x32_10(c, (uint32_t)seed, (uint32_t)(seed >> 32));
     box_muller(w.x[0], w.x[1], z[4 * b], z[4 * b + 1]);
-    box_muller(w.x[2], w.x[3], z[4 * b + 2], z[4 * b + 3]);
+    if (2 * b + 1 < pairs) box_muller(w.x[2], w.x[3], z[4 * b + 2], z[4 * b + 3]);
   }
   // upper-triangle position of (i, j), i <= j: (0,0) 0, (0,1) 1, (0,2) 2, (1,1) 3, (1,2) 4, (2,2) 5
   constexpr int UP[9] = {0, 1, 2, 1, 3, 4, 2, 4, 5};
