@@ -360,3 +360,35 @@ def test_fused_voigt_equals_matrix_form_and_oracle(n):
     assert normwise(psi[:, None], rpsi[:, None]).max() <= 1e-13
     assert normwise(g, rg).max() <= 1e-13
     assert normwise(K.reshape(n, 36), rK).max() <= 1e-13
+
+
+@pytest.mark.parametrize("which", ["smf", "gmf"])
+def test_fused_step_replays_in_a_graph(which):
+    """the fused entry points allocate nothing and record no memset: captured in a graph, every replay (with eager
+    work on the outputs between replays) gives the eager results bitwise"""
+    import torch
+    lib = matfun_amd.library()
+    n = 777
+    X = spd_pos(n, 60) if which == "smf" else general_pos(n, 61)
+    fn = lib.smf_hip_hencky_matrix if which == "smf" else lib.gmf_hip_hencky
+    dX = torch.from_numpy(X).cuda()
+    outs = [torch.empty(s, dtype=torch.float64, device="cuda") for s in ((n, 9), (n,), (n, 9), (n, 81))]
+    assert fn(n, dX.data_ptr(), *[o.data_ptr() for o in outs], torch.cuda.current_stream().cuda_stream) == 0
+    torch.cuda.synchronize()
+    ref = [o.clone() for o in outs]
+    graph = torch.cuda.CUDAGraph()
+    side = torch.cuda.Stream()
+    side.wait_stream(torch.cuda.current_stream())
+    with torch.cuda.stream(side):
+        with torch.cuda.graph(graph):
+            rc = fn(n, dX.data_ptr(), *[o.data_ptr() for o in outs], torch.cuda.current_stream().cuda_stream)
+    torch.cuda.current_stream().wait_stream(side)
+    assert rc == 0
+    for _ in range(3):
+        for o in outs:
+            o.fill_(7.0)                           # (eager work between the replays)
+        torch.cuda.synchronize()
+        graph.replay()
+        torch.cuda.synchronize()
+        for o, r in zip(outs, ref):
+            assert torch.equal(torch.nan_to_num(o, nan=-1.0), torch.nan_to_num(r, nan=-1.0))
