@@ -35,8 +35,10 @@ MF_DEV void box_muller(uint32_t w1, uint32_t w2, double& z0, double& z1) {
   const double TWO_PI = 6.283185307179586;
   const double u1 = ((double)w1 + 0.5) * 2.3283064365386963e-10, u2 = ((double)w2 + 0.5) * 2.3283064365386963e-10;
   const double r = sqrt(-2.0 * log(u1));
-  z0 = r * cos(TWO_PI * u2);
-  z1 = r * sin(TWO_PI * u2);
+  double sn, cs;
+  sincos(TWO_PI * u2, &sn, &cs);                 // (one shared argument reduction)
+  z0 = r * cs;
+  z1 = r * sn;
 }
 
 // the relative perturbation factors 1 + sigma z of the 9 entries of matrix m at a Newton iteration, z from
