@@ -265,8 +265,9 @@ def main():
     ap.add_argument("--no-tune", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(json.dumps({"error": "this build runs on one GPU only; --gpus must be 1 (the batch shards trivially "
-                                   "by contiguous matrix ranges, one process per GPU; see DESIGN.md)"}))
+        print(json.dumps({"error": "this benchmark runs on one GPU only; --gpus must be 1 (the batch shards "
+                                   "trivially by contiguous matrix ranges, one process per GPU, with no "
+                                   "communication: every matrix is independent)"}))
         sys.exit(2)
 
     import torch
