@@ -105,7 +105,7 @@ typedef struct mf_hip_plan mf_hip_plan;          /* opaque */
 typedef struct mf_hip_sequence mf_hip_sequence;  /* opaque */
 
 /* Host set-up (allocates device tables: tiles of up to 64 matrices of one class, per-matrix rows, series tables;
- * synchronous).  Returns 0, -1 invalid groups, -2 device error. */
+ * synchronous).  Returns 0, -1 invalid groups (or 2^31 matrices or more in all), -2 device error. */
 int mf_hip_plan_create(const mf_hip_group *groups, int64_t ngroups, mf_hip_plan **plan);
 int mf_hip_plan_destroy(mf_hip_plan *plan);
 /* sizes of the batch: what 0 = matrices N, 1 = J rows, 2 = H rows, 3 = tiles, 4 = classes present */

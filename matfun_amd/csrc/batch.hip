@@ -166,8 +166,9 @@ extern "C" int mf_hip_plan_create(const mf_hip_group* groups, int64_t ngroups, m
   if (!p) return -2;
   for (int64_t g = 0; g < ngroups; g++) {
     const mf_hip_group& G = groups[g];
+    // (fewer than 2^31 matrices in all: the batch kernels may keep input rows as 32-bit values)
     if (G.family < 0 || G.family > 1 || G.kind < EXP || G.kind > SQRT || G.order < 0 || G.order > 2 ||
-        G.count < 0 || (G.kind == POW && !std::isfinite(G.alpha))) {
+        G.count < 0 || (G.kind == POW && !std::isfinite(G.alpha)) || p->N + G.count > INT32_MAX) {
       delete p;
       return -1;
     }

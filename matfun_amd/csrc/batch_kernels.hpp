@@ -66,12 +66,12 @@ void batch_launch_gmf(int cls, hipStream_t st, unsigned grid, bool loop, const B
 static __constant__ BatchTables g_tab;
 
 // one tile of class CLS (lane and the table pointers opaque per tile)
-template <int CLS, class OT>
-MF_DEV void run_tile(const BatchArgs& a, const BatchTile& tl, const ListRows& rows, OT* F, OT* J, OT* H,
+template <int CLS, class OT, class R = ListRows>
+MF_DEV void run_tile(const BatchArgs& a, const BatchTile& tl, const R& rows, OT* F, OT* J, OT* H,
                      unsigned char* lds, const short* lutJ, const short* lutH) {
   constexpr int FAM = cls_fam(CLS), SLOT = cls_slot(CLS), ORDER = cls_order(CLS);
   const int lane = opaque((int)threadIdx.x);
-  if (FAM == 0) {
+  if constexpr (FAM == 0) {
     constexpr int KIND = SLOT == 0 ? EXP : (SLOT == 1 ? LOG : (SLOT == 2 ? SQRT : POW));
     const double* c2 = opaque(g_tab.smf_c2[SLOT < 3 ? SLOT : 0]);
     smf_tile<KIND, true, ORDER, OT, 1, SLOT == SLOT_POW>(rows, tl.nm, a.A, F, J, H, SLOT == 2 ? 0.5 : 0.0,
@@ -91,6 +91,25 @@ MF_DEV void stage_rows(const BatchArgs& a, const BatchTile& tl, int64_t* rows, i
     rows[2 * WAVE + lane] = a.rh[tl.off + lane];
   }
   __syncthreads();
+}
+
+// ListRows32 of a tile: its input rows into LDS (32-bit: a plan holds fewer than 2^31 matrices), the run masks of
+// the three regions by ballot
+MF_DEV ListRows32 stage_rows32(const BatchArgs& a, const BatchTile& tl, int32_t* rows, int lane) {
+  const bool in = lane < tl.nm;
+  int64_t m = 0, jr = 0, hr = 0;
+  if (in) {
+    m = a.rm[tl.off + lane];
+    jr = a.rj[tl.off + lane];
+    hr = a.rh[tl.off + lane];
+    rows[lane] = (int32_t)m;
+  }
+  const int64_t pm = __shfl_up(m, 1), pj = __shfl_up(jr, 1), ph = __shfl_up(hr, 1);
+  const bool on = in && lane > 0;
+  const ListRows32 R{rows, a.rj + tl.off, a.rh + tl.off,
+                     {__ballot(on && m == pm + 1), __ballot(on && jr == pj + 1), __ballot(on && hr == ph + 1)}};
+  __syncthreads();
+  return R;
 }
 
 template <int C, class OT>
@@ -134,14 +153,21 @@ batch_class_kernel(BatchArgs a, OT* __restrict__ F, OT* __restrict__ J, OT* __re
   constexpr int B = FAM == 0 ? SmfLds<true, ORDER, OT>::BYTES : GmfLds<ORDER, OT>::BYTES;
   __shared__ __attribute__((aligned(16))) unsigned char lds[B];
   __shared__ short lutJ[FAM == 0 && ORDER >= 1 ? 81 + LUT_PAD : 1], lutH[FAM == 0 && ORDER >= 2 ? 729 + LUT_PAD : 1];
-  __shared__ int64_t rows[3 * WAVE];
+  // the FP64 gmf order-2 tile leaves 304 B of LDS under the eight-workgroups-per-CU limit (20176 of 20480 B):
+  // its rows are staged as ListRows32 (256 B), the other classes' as ListRows (1.5 KB)
+  constexpr bool R32 = FAM == 1 && ORDER == 2 && sizeof(OT) == 8;
+  __shared__ int64_t rows[R32 ? 1 : 3 * WAVE];
+  __shared__ int32_t rows32[R32 ? WAVE : 1];
   const int lane = threadIdx.x;
   if (FAM == 0) smf_luts<true, ORDER, OT>(lutJ, lutH, lane);
-  const ListRows R{rows, rows + WAVE, rows + 2 * WAVE};
   for (int64_t t = a.t0 + blockIdx.x; t < a.t1; t += gridDim.x) {
     const BatchTile tl = a.tiles[t];
-    stage_rows(a, tl, rows, lane);
-    run_tile<CLS, OT>(a, tl, R, F, J, H, lds, lutJ, lutH);
+    if constexpr (R32) {
+      run_tile<CLS, OT>(a, tl, stage_rows32(a, tl, rows32, lane), F, J, H, lds, lutJ, lutH);
+    } else {
+      stage_rows(a, tl, rows, lane);
+      run_tile<CLS, OT>(a, tl, ListRows{rows, rows + WAVE, rows + 2 * WAVE}, F, J, H, lds, lutJ, lutH);
+    }
     if (!LOOP) break;
   }
 }

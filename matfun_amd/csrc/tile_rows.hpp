@@ -13,7 +13,7 @@ namespace mf {
 constexpr int WAVE = 64;
 
 struct RunRows {
-  static constexpr bool RUN = true;
+  static constexpr bool RUN = true, MASKS = false;
   int64_t base;
   MF_DEV int64_t m(int j) const { return base + j; }
   MF_DEV int64_t jr(int j) const { return base + j; }
@@ -21,11 +21,25 @@ struct RunRows {
 };
 
 struct ListRows {
-  static constexpr bool RUN = false;
+  static constexpr bool RUN = false, MASKS = false;
   const int64_t* mr;              // LDS
   const int64_t* jrr;
   const int64_t* hrr;
   MF_DEV int64_t m(int j) const { return mr[j]; }
+  MF_DEV int64_t jr(int j) const { return jrr[j]; }
+  MF_DEV int64_t hr(int j) const { return hrr[j]; }
+};
+
+// rows listed per matrix with less LDS (for a tile body whose own LDS leaves no room for three 64-bit rows per
+// matrix): input / F rows 32-bit in LDS, J and H rows read from the batch's tables in global memory at the start of
+// each run, and per region (Region below) a mask of where the rows continue (bit j: row j is row j - 1 plus one)
+struct ListRows32 {
+  static constexpr bool RUN = false, MASKS = true;
+  const int32_t* mr;              // LDS
+  const int64_t* jrr;             // global
+  const int64_t* hrr;
+  uint64_t cont[3];
+  MF_DEV int64_t m(int j) const { return (int64_t)mr[j]; }
   MF_DEV int64_t jr(int j) const { return jrr[j]; }
   MF_DEV int64_t hr(int j) const { return hrr[j]; }
 };
@@ -53,6 +67,9 @@ MF_DEV const T* opaque(const T* p) {
 
 // the rows of tile matrices p0, p0 + 1, ... (a pass over part of the tile)
 MF_DEV RunRows shift_rows(const RunRows& r, int p0) { return RunRows{r.base + p0}; }
+MF_DEV ListRows32 shift_rows(const ListRows32& r, int p0) {
+  return ListRows32{r.mr + p0, r.jrr + p0, r.hrr + p0, {r.cont[0] >> p0, r.cont[1] >> p0, r.cont[2] >> p0}};
+}
 MF_DEV ListRows shift_rows(const ListRows& r, int p0) { return ListRows{r.mr + p0, r.jrr + p0, r.hrr + p0}; }
 
 // which row of a tile matrix an output region uses
@@ -107,8 +124,13 @@ MF_DEV void copy_out(OT* __restrict__ out, const R& rows, int count, int lane, G
 // consecutive (a batch's groups of matrices), and a run's outputs form one contiguous block.
 template <int REG, class R, class Fn>
 __device__ void for_row_runs(const R& rows, int cnt, int lane, Fn fn) {
-  const bool c = lane > 0 && lane < cnt && row_of<REG>(rows, lane) == row_of<REG>(rows, lane - 1) + 1;
-  const uint64_t cont = __ballot(c);
+  uint64_t cont;
+  if constexpr (R::MASKS) {
+    cont = rows.cont[REG];
+  } else {
+    const bool c = lane > 0 && lane < cnt && row_of<REG>(rows, lane) == row_of<REG>(rows, lane - 1) + 1;
+    cont = __ballot(c);
+  }
   for (int j0 = 0; j0 < cnt;) {
     const uint64_t stop = j0 + 1 < 64 ? (~cont & (~0ull << (j0 + 1))) : 0ull;
     int j1 = stop ? __ffsll((long long)stop) - 1 : 64;
