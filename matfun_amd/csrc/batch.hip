@@ -159,19 +159,23 @@ int slot_of(int family, int kind) {
 extern "C" int mf_hip_plan_create(const mf_hip_group* groups, int64_t ngroups, mf_hip_plan** out) {
   if (!out || ngroups < 0 || (ngroups > 0 && !groups)) return -1;
   *out = nullptr;
-  // ---- validate; rows of each matrix in the caller's layout; matrices per class in group order
+  // ---- validate every group first (fewer than 2^31 matrices in all: the batch kernels may keep input rows as
+  //      32-bit values), then the rows of each matrix in the caller's layout, matrices per class in group order
+  int64_t total = 0;
+  for (int64_t g = 0; g < ngroups; g++) {
+    const mf_hip_group& G = groups[g];
+    if (G.family < 0 || G.family > 1 || G.kind < EXP || G.kind > SQRT || G.order < 0 || G.order > 2 ||
+        G.count < 0 || (G.kind == POW && !std::isfinite(G.alpha)))
+      return -1;
+    total += G.count;
+    if (total > INT32_MAX) return -1;
+  }
   std::vector<std::vector<int64_t>> cm(NCLS), cj(NCLS), ch(NCLS);
   std::vector<double> galpha;
   mf_hip_plan* p = new (std::nothrow) mf_hip_plan;
   if (!p) return -2;
   for (int64_t g = 0; g < ngroups; g++) {
     const mf_hip_group& G = groups[g];
-    // (fewer than 2^31 matrices in all: the batch kernels may keep input rows as 32-bit values)
-    if (G.family < 0 || G.family > 1 || G.kind < EXP || G.kind > SQRT || G.order < 0 || G.order > 2 ||
-        G.count < 0 || (G.kind == POW && !std::isfinite(G.alpha)) || p->N + G.count > INT32_MAX) {
-      delete p;
-      return -1;
-    }
     const int c = cls_of(G.family, slot_of(G.family, G.kind), G.order);
     for (int32_t i = 0; i < G.count; i++) {
       cm[c].push_back(p->N + i);

@@ -97,3 +97,22 @@ def test_hencky_entry_points_reject_before_device_work():
         assert f(10, None, dummy, None, None, None, None) == -1                 # outputs without A
         assert f(0, None, None, None, None, None, None) == 0
         assert f(10, dummy, None, None, None, None, None) == 0                  # no output asked for
+
+
+@pytest.mark.skipif(not os.path.exists(LIB), reason="HIP library not built (python -m matfun_amd.build)")
+def test_plan_rejects_invalid_groups_and_2_31_matrices_before_device_work():
+    """mf_hip_plan_create validates every group before it builds anything: an invalid group, or 2^31 matrices or more
+    in all (the batch kernels may keep input rows as 32-bit values), give -1 and no plan"""
+    class Group(ctypes.Structure):
+        _fields_ = [("family", ctypes.c_int32), ("kind", ctypes.c_int32), ("order", ctypes.c_int32),
+                    ("count", ctypes.c_int32), ("alpha", ctypes.c_double)]
+    lib = ctypes.CDLL(LIB)
+    f = lib.mf_hip_plan_create
+    f.argtypes, f.restype = [ctypes.POINTER(Group), ctypes.c_int64, ctypes.POINTER(ctypes.c_void_p)], ctypes.c_int
+    plan = ctypes.c_void_p(0x1234)
+    big = (Group * 2)(Group(0, 1, 2, 2 ** 30, 0.0), Group(1, 0, 1, 2 ** 30, 0.0))
+    assert f(big, 2, ctypes.byref(plan)) == -1 and not plan.value
+    bad = (Group * 2)(Group(0, 1, 2, 10, 0.0), Group(2, 0, 1, 10, 0.0))           # family 2
+    assert f(bad, 2, ctypes.byref(plan)) == -1 and not plan.value
+    neg = (Group * 1)(Group(1, 2, 1, 10, float("nan")))                            # pow with a NaN exponent
+    assert f(neg, 1, ctypes.byref(plan)) == -1 and not plan.value
