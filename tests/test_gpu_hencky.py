@@ -392,3 +392,27 @@ def test_fused_step_replays_in_a_graph(which):
         torch.cuda.synchronize()
         for o, r in zip(outs, ref):
             assert torch.equal(torch.nan_to_num(o, nan=-1.0), torch.nan_to_num(r, nan=-1.0))
+
+
+def test_voigt_newton_converges_quadratically():
+    """Newton's method in Voigt coordinates for g(v) = T on symmetric positive definite matrices with smf_hip_hencky's
+    6 x 6 tangent (the form an FE code assembles): quadratic convergence to the rounding level (the oracle on the
+    CPU, same draws: 1.1e-3, 4.2e-6, 1.4e-10, 1.8e-15 at the 99th percentile)"""
+    import torch
+    n = 4096
+    gen = torch.Generator(device="cuda").manual_seed(14)
+    M = 0.1 * torch.randn((n, 3, 3), dtype=torch.float64, device="cuda", generator=gen)
+    Xs = 2.0 * torch.eye(3, dtype=torch.float64, device="cuda") + 0.5 * (M + M.transpose(1, 2))
+    vs = Xs.reshape(n, 9)[:, [0, 4, 8, 1, 2, 5]].contiguous()
+    T = matfun_amd.smf_hencky(vs)[2]
+    v = (vs + 1e-3 * torch.randn((n, 6), dtype=torch.float64, device="cuda", generator=gen)).contiguous()
+    res = []
+    for _ in range(5):
+        _, _, g, K = matfun_amd.smf_hencky(v)
+        r = g - T
+        res.append(torch.quantile(r.norm(dim=1) / T.norm(dim=1), 0.99).item())
+        v = (v - torch.linalg.solve_ex(K, r.unsqueeze(-1))[0].squeeze(-1)).contiguous()
+    assert res[-1] <= 1e-13, res
+    for a, b in zip(res, res[1:]):
+        if a > 1e-9:
+            assert b <= 50.0 * a * a + 1e-13, res
