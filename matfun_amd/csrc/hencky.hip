@@ -163,10 +163,10 @@ smf_hencky_kernel(int64_t n, const double* __restrict__ x, double* __restrict__ 
 }
 
 // LDS of the fused gmf kernel, in doubles: r1, r2, D3 per matrix [64][28] (as the gmf tile), log X [64][9], the J tile
-// of a group of six matrices [6][81] (the X1 tables of the two H lane groups [2][243] once K = J^T J is formed), the K
-// tile of the group [6][81]
+// of a group of six matrices [6][81] (the X1 tables of the two H lane groups [2][243] once K = J^T J is formed):
+// 22.8 KB, seven workgroups per CU (K goes to its output and is completed there)
 struct GhLds {
-  static constexpr int MS = 28, GM = 6, FB = WAVE * MS, JB = FB + WAVE * 9, KB = JB + GM * 81, TOT = KB + GM * 81;
+  static constexpr int MS = 28, GM = 6, FB = WAVE * MS, JB = FB + WAVE * 9, TOT = JB + GM * 81;
   static_assert(TOT >= WAVE * 9 && GM * 81 >= 2 * 243, "the input block overlays the start; X1 tables in the J tile");
 };
 
@@ -181,7 +181,6 @@ __global__ void __launch_bounds__(WAVE) gmf_hencky_kernel(int64_t n, const doubl
   double* mainr = lds;
   double* fb = lds + GhLds::FB;
   double* jt = lds + GhLds::JB;
-  double* kt = lds + GhLds::KB;
   load_in<9>(lds, A, RunRows{base}, nm, lane);
   wave_sync();
   double a[9];
@@ -236,13 +235,14 @@ __global__ void __launch_bounds__(WAVE) gmf_hencky_kernel(int64_t n, const doubl
       }
     }
     wave_sync();                                   // (J tile staged)
-    for (int t = lane; t < 81 * cg; t += WAVE) {   // K = J^T J of the group
+    double* Kg = K ? K + (base + p0) * 81 : nullptr;   // (the group's K block)
+    for (int t = lane; K && t < 81 * cg; t += WAVE) {   // K = J^T J of the group, stored (consecutive t)
       const int m = t / 81, ce = t - 81 * m, c = ce / 9, ee = ce - 9 * c;
       const double* jm = jt + 81 * m;
       double acc = 0.0;
 #pragma unroll
       for (int q = 0; q < 9; q++) acc = fma(jm[9 * q + c], jm[9 * q + ee], acc);
-      kt[t] = acc;
+      Kg[t] = acc;
     }
     if (g && lane < 9 * cg) {                      // g = J^T L, stored at once (consecutive lanes, consecutive g)
       const int m = lane / 9, c = lane - 9 * m;
@@ -251,7 +251,7 @@ __global__ void __launch_bounds__(WAVE) gmf_hencky_kernel(int64_t n, const doubl
       for (int q = 0; q < 9; q++) acc = fma(fb[(p0 + m) * 9 + q], jt[81 * m + 9 * q + c], acc);
       g[(base + p0) * 9 + lane] = acc;
     }
-    wave_sync();                                   // (J tile read: it takes the X1 tables)
+    wave_sync();                                   // (J tile read: it takes the X1 tables; J^T J stored and visible)
     for (int h0 = 0; h0 < cg; h0 += 2) {
       const int cnt = cg - h0 < 2 ? cg - h0 : 2;
       const int mh = h0 + (gh < 2 ? gh : 0);       // the lane group's matrix in the group
@@ -267,19 +267,19 @@ __global__ void __launch_bounds__(WAVE) gmf_hencky_kernel(int64_t n, const doubl
       wave_sync();
       if (gh < cnt) {
         gmf_h_out(sh, sh + 9, s2, d, e, f, y, ys, out);
-        const double* lm = fb + (p0 + mh) * 9;     // K += L : H on this lane's entries (c, Lg)
+        const double* lm = fb + (p0 + mh) * 9;     // K += L : H on this lane's entries (c, Lg), in the output
+        if (K) {
 #pragma unroll
-        for (int c = 0; c < 3; c++) {
-          double acc = kt[mh * 81 + 27 * c + Lg];
+          for (int c = 0; c < 3; c++) {
+            double acc = Kg[mh * 81 + 27 * c + Lg];
 #pragma unroll
-          for (int q = 0; q < 9; q++) acc = fma(lm[q], out[3 * q + c], acc);
-          kt[mh * 81 + 27 * c + Lg] = acc;
+            for (int q = 0; q < 9; q++) acc = fma(lm[q], out[3 * q + c], acc);
+            Kg[mh * 81 + 27 * c + Lg] = acc;
+          }
         }
       }
       wave_sync();                                 // (X1 tables read before the next pass writes them)
     }
-    if (K) copy_block<2>(K + (base + p0) * 81, kt, 81 * cg, lane);
-    wave_sync();                                   // (K tile read before the next group writes it)
   }
 }
 
