@@ -162,11 +162,12 @@ smf_hencky_kernel(int64_t n, const double* __restrict__ x, double* __restrict__ 
   }
 }
 
-// LDS of the fused gmf kernel, in doubles: r1, r2, D3 per matrix [64][28] (as the gmf tile), log X [64][9], the J tile
-// of a group of six matrices [6][81] (the X1 tables of the two H lane groups [2][243] once K = J^T J is formed):
-// 22.8 KB, seven workgroups per CU (K goes to its output and is completed there)
+// LDS of the fused gmf kernel, in doubles: r1, r2, D3 per matrix [64][28] (as the gmf tile), the J tile of a group
+// of six matrices [6][81] (the X1 tables of the two H lane groups [2][243] once K = J^T J is formed): 18.2 KB, eight
+// workgroups per CU (K goes to its output and is completed there; log X stays in its lane's registers and reaches
+// the J / H lanes by shuffles)
 struct GhLds {
-  static constexpr int MS = 28, GM = 6, FB = WAVE * MS, JB = FB + WAVE * 9, TOT = JB + GM * 81;
+  static constexpr int MS = 28, GM = 6, JB = WAVE * MS, TOT = JB + GM * 81;
   static_assert(TOT >= WAVE * 9 && GM * 81 >= 2 * 243, "the input block overlays the start; X1 tables in the J tile");
 };
 
@@ -179,7 +180,6 @@ __global__ void __launch_bounds__(WAVE) gmf_hencky_kernel(int64_t n, const doubl
   const int nm = (int)(n - base < WAVE ? n - base : WAVE);
   const int lane = threadIdx.x;
   double* mainr = lds;
-  double* fb = lds + GhLds::FB;
   double* jt = lds + GhLds::JB;
   load_in<9>(lds, A, RunRows{base}, nm, lane);
   wave_sync();
@@ -202,10 +202,7 @@ __global__ void __launch_bounds__(WAVE) gmf_hencky_kernel(int64_t n, const doubl
   }
   double ps = 0.0;
 #pragma unroll
-  for (int k = 0; k < 9; k++) {
-    fb[lane * 9 + k] = o.F[k];
-    ps = fma(o.F[k], o.F[k], ps);
-  }
+  for (int k = 0; k < 9; k++) ps = fma(o.F[k], o.F[k], ps);
   double d2r[7];                                   // D2 (6 distinct, by sym3) and sc1 of this lane's matrix
 #pragma unroll
   for (int i = 0; i < 3; i++)
@@ -214,16 +211,21 @@ __global__ void __launch_bounds__(WAVE) gmf_hencky_kernel(int64_t n, const doubl
   d2r[6] = o.sc1;
   const double sc2 = o.sc2;
   wave_sync();
-  if (Lo) copy_block<2>(Lo + base * 9, fb, 9 * nm, lane);
+  if (Lo && lane < nm) {                           // (log X from the lanes: 72 B per lane)
+#pragma unroll
+    for (int k = 0; k < 9; k++) Lo[(base + lane) * 9 + k] = o.F[k];
+  }
   if (psi && lane < nm) psi[base + lane] = 0.5 * ps;
   const int gj = lane / 9, Lj = lane - 9 * gj;     // J: lane (gj, Lj = 3 c + d), gj < 6
   const int gh = lane / 27, Lg = lane - 27 * gh;   // H: lane (gh, Lg = 9 d + 3 e + f), gh < 2
   const int d = Lg / 9, e = (Lg / 3) % 3, f = Lg % 3;
   for (int p0 = 0; p0 < nm; p0 += GM) {            // groups of six matrices: J, K = J^T J, g, then K += L : H
     const int cg = nm - p0 < GM ? nm - p0 : GM;
-    double dj[7];                                  // (from the matrix's lane; every lane takes part)
+    double dj[7], lj[9];                           // (from the matrix's lane; every lane takes part)
 #pragma unroll
     for (int k = 0; k < 7; k++) dj[k] = __shfl(d2r[k], (p0 + (gj < GM ? gj : 0)) & (WAVE - 1));
+#pragma unroll
+    for (int k = 0; k < 9; k++) lj[k] = __shfl(o.F[k], (p0 + (gj < GM ? gj : 0)) & (WAVE - 1));
     if (gj < cg) {
       const double* s = mainr + (p0 + gj) * GhLds::MS;
       double out[3];
@@ -245,10 +247,10 @@ __global__ void __launch_bounds__(WAVE) gmf_hencky_kernel(int64_t n, const doubl
       Kg[t] = acc;
     }
     if (g && lane < 9 * cg) {                      // g = J^T L, stored at once (consecutive lanes, consecutive g)
-      const int m = lane / 9, c = lane - 9 * m;
+      const int m = lane / 9, c = lane - 9 * m;    // (m = gj: lj is this matrix's L)
       double acc = 0.0;
 #pragma unroll
-      for (int q = 0; q < 9; q++) acc = fma(fb[(p0 + m) * 9 + q], jt[81 * m + 9 * q + c], acc);
+      for (int q = 0; q < 9; q++) acc = fma(lj[q], jt[81 * m + 9 * q + c], acc);
       g[(base + p0) * 9 + lane] = acc;
     }
     wave_sync();                                   // (J tile read: it takes the X1 tables; J^T J stored and visible)
@@ -256,6 +258,9 @@ __global__ void __launch_bounds__(WAVE) gmf_hencky_kernel(int64_t n, const doubl
       const int cnt = cg - h0 < 2 ? cg - h0 : 2;
       const int mh = h0 + (gh < 2 ? gh : 0);       // the lane group's matrix in the group
       const double s2 = __shfl(sc2, (p0 + mh) & (WAVE - 1));
+      double lm[9];                                // (L of the lane group's matrix, from its lane)
+#pragma unroll
+      for (int k = 0; k < 9; k++) lm[k] = __shfl(o.F[k], (p0 + mh) & (WAVE - 1));
       const double* sh = mainr + (p0 + mh) * GhLds::MS;
       double* ys = jt + (gh < 2 ? gh : 0) * 243;
       double y[9], out[27];
@@ -267,8 +272,7 @@ __global__ void __launch_bounds__(WAVE) gmf_hencky_kernel(int64_t n, const doubl
       wave_sync();
       if (gh < cnt) {
         gmf_h_out(sh, sh + 9, s2, d, e, f, y, ys, out);
-        const double* lm = fb + (p0 + mh) * 9;     // K += L : H on this lane's entries (c, Lg), in the output
-        if (K) {
+        if (K) {                                   // K += L : H on this lane's entries (c, Lg), in the output
 #pragma unroll
           for (int c = 0; c < 3; c++) {
             double acc = Kg[mh * 81 + 27 * c + Lg];
