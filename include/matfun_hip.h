@@ -105,7 +105,10 @@ typedef struct mf_hip_plan mf_hip_plan;          /* opaque */
 typedef struct mf_hip_sequence mf_hip_sequence;  /* opaque */
 
 /* Host set-up (allocates device tables: tiles of up to 64 matrices of one class, per-matrix rows, series tables;
- * synchronous).  Returns 0, -1 invalid groups (or 2^31 matrices or more in all), -2 device error. */
+ * synchronous).  Returns 0, -1 invalid groups (or 2^31 matrices or more in all), -2 device error.
+ * A plan must outlive the sequences captured from it (their graphs use its tables): mf_hip_plan_destroy returns -1
+ * and frees nothing while a sequence of the plan is not yet destroyed (0 otherwise; NULL is a no-op).  The caller
+ * makes sure no evaluation on the plan is still queued when it is destroyed. */
 int mf_hip_plan_create(const mf_hip_group *groups, int64_t ngroups, mf_hip_plan **plan);
 int mf_hip_plan_destroy(mf_hip_plan *plan);
 /* sizes of the batch: what 0 = matrices N, 1 = J rows, 2 = H rows, 3 = tiles, 4 = classes present */
@@ -142,7 +145,8 @@ int mf_hip_perturb(const mf_hip_plan *plan, double *A, double rel_sigma, uint64_
 /* A sequence of Newton iterations captured in one hipGraph: A <- A0 (device copy), then `iterations` times
  * { mf_hip_perturb(iteration i = 1..iterations); mf_hip_batch_eval(_mixed) } with the plan's strategy at creation.
  * F / J / H hold the last iteration's results.  Replays are independent (each starts from A0).  Creation captures and
- * instantiates (synchronous); launch is asynchronous on `stream`. */
+ * instantiates (synchronous); launch is asynchronous on `stream`.  The sequence keeps using the plan's tables and the
+ * buffers given here: destroy it before the plan (mf_hip_plan_destroy refuses until then) and keep the buffers. */
 int mf_hip_sequence_create(const mf_hip_plan *plan, const double *A0, double *A, const double *alpha, void *F, void *J,
                            void *H, int mixed, int iterations, double rel_sigma, uint64_t seed,
                            mf_hip_sequence **seq);

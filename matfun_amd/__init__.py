@@ -17,6 +17,7 @@ alpha = 0 gives I, alpha < 0 powers of the inverse, a non-finite exponent NaN re
 import ctypes
 import math
 import os
+import weakref
 
 __version__ = "0.1.3"
 __all__ = ["smf", "smf_packed", "expand_packed", "gmf", "hencky", "smf_hencky", "gmf_hencky", "Batch", "Group",
@@ -323,6 +324,53 @@ def _batch_lib():
 FAMILIES = {"smf": 0, "gmf": 1}
 
 
+def _check_layout(t, name, dtype, count, exact):
+    """t must be a contiguous tensor of `dtype` with `count` elements (exact) or at least `count`; the C ABI takes raw
+    device pointers, so a tensor that is too short, strided or of another type would be read or written out of
+    bounds by the kernels.  Returns the tensor's device (checked by the caller once every tensor passed this)."""
+    import torch
+    if not isinstance(t, torch.Tensor):
+        raise ValueError(f"{name}: expected a torch CUDA tensor, got {type(t).__name__}")
+    if t.dtype != dtype:
+        raise ValueError(f"{name}: expected dtype {dtype}, got {t.dtype}")
+    if not t.is_contiguous():
+        raise ValueError(f"{name}: expected a contiguous tensor")
+    if (t.numel() != count) if exact else (t.numel() < count):
+        raise ValueError(f"{name}: {t.numel()} elements, expected {'' if exact else 'at least '}{count}")
+    return name, t.device
+
+
+def _check_batch_args(n, nj, nh, A=None, alpha=None, out=None, precision="fp64", names=("A",)):
+    """checks every tensor a batch call passes to the C ABI before anything is launched: the inputs named in
+    `names` (float64, n x 9 elements each), alpha (None or n float64), out = (F, J, H) in the precision's type with
+    room for n x 9, nj x 81 and nh x 729 elements (J / H may be None when the batch has no row for them); then that
+    all of them are on one GPU.  Raises ValueError."""
+    import torch
+    if precision not in ("fp64", "mixed"):
+        raise ValueError("precision must be 'fp64' or 'mixed'")
+    ins = A if isinstance(A, (tuple, list)) else (A,)
+    placed = []
+    for t, name in zip(ins, names):
+        if n == 0 and (t is None or (isinstance(t, torch.Tensor) and t.numel() == 0)):
+            continue                                   # (an empty batch reads nothing)
+        placed.append(_check_layout(t, name, torch.float64, 9 * n, exact=True))
+    if alpha is not None:
+        placed.append(_check_layout(alpha, "alpha", torch.float64, n, exact=True))
+    if out is not None:
+        dt = torch.float32 if precision == "mixed" else torch.float64
+        for t, name, count in zip(out, ("F", "J", "H"), (9 * n, 81 * nj, 729 * nh)):
+            if t is None or t.numel() == 0:
+                if count > 0:
+                    raise ValueError(f"{name}: the batch has {count} elements of it, got no tensor")
+                continue
+            placed.append(_check_layout(t, name, dt, count, exact=False))
+    for name, dev in placed:
+        if dev.type != "cuda":
+            raise ValueError(f"{name}: expected a tensor on the GPU, got one on {dev}")
+    if len({dev for _, dev in placed}) > 1:
+        raise ValueError(f"tensors on different devices: {sorted(str(d) for _, d in placed)}")
+
+
 class Batch:
     """A heterogeneous batch: groups (family, kind, order, count[, alpha]) of 3x3 matrices, evaluated by one kernel.
 
@@ -357,10 +405,16 @@ class Batch:
         self._h, self._lib = h, lib
         self.n, self.nj, self.nh, self.ntiles, self.nclasses = (int(lib.mf_hip_plan_size(h, w)) for w in range(5))
         self.groups = groups
+        self._sequences = weakref.WeakSet()
 
     def close(self):
+        """frees the plan; refused (RuntimeError) while a Sequence of this batch is open, since its graph uses the
+        plan's device tables"""
         if self._h:
-            self._lib.mf_hip_plan_destroy(self._h)
+            if any(s._h for s in list(self._sequences)):
+                raise RuntimeError("Batch.close: close the batch's open sequences first")
+            if self._lib.mf_hip_plan_destroy(self._h) != 0:
+                raise RuntimeError("mf_hip_plan_destroy refused: a sequence of the plan is still alive")
             self._h = None
 
     def __del__(self):
@@ -380,8 +434,14 @@ class Batch:
     def _p(t):
         return None if t is None or t.numel() == 0 else t.data_ptr()
 
+    def _live(self):
+        if not self._h:
+            raise RuntimeError("the batch is closed")
+
     def eval(self, A, alpha=None, out=None, precision="fp64"):
         import torch
+        self._live()
+        _check_batch_args(self.n, self.nj, self.nh, A, alpha, out, precision)
         F, J, H = out if out is not None else self.outputs(precision)
         fn = self._lib.mf_hip_batch_eval_mixed if precision == "mixed" else self._lib.mf_hip_batch_eval
         r = fn(self._h, self._p(A), self._p(alpha), self._p(F), self._p(J), self._p(H),
@@ -396,6 +456,8 @@ class Batch:
 
     def tune(self, A, alpha=None, out=None, precision="fp64"):
         import torch
+        self._live()
+        _check_batch_args(self.n, self.nj, self.nh, A, alpha, out, precision)
         F, J, H = out if out is not None else self.outputs(precision)
         res = (ctypes.c_double * 4)()
         r = self._lib.mf_hip_plan_tune(self._h, self._p(A), self._p(alpha), self._p(F), self._p(J), self._p(H),
@@ -406,6 +468,8 @@ class Batch:
 
     def perturb(self, A, sigma, seed, iteration):
         import torch
+        self._live()
+        _check_batch_args(self.n, self.nj, self.nh, A)
         r = self._lib.mf_hip_perturb(self._h, self._p(A), float(sigma), int(seed), int(iteration),
                                      torch.cuda.current_stream().cuda_stream)
         if r != 0:
@@ -413,7 +477,11 @@ class Batch:
 
     def sequence(self, A0, A, out, iterations, sigma, seed, alpha=None, precision="fp64"):
         """a graph-captured Newton sequence (mf_hip_sequence_*): .launch() replays it on the current stream"""
-        return Sequence(self, A0, A, out, iterations, sigma, seed, alpha, precision)
+        self._live()
+        _check_batch_args(self.n, self.nj, self.nh, (A0, A), alpha, out, precision, names=("A0", "A"))
+        s = Sequence(self, A0, A, out, iterations, sigma, seed, alpha, precision)
+        self._sequences.add(s)
+        return s
 
 
 def group_array(family, kind, order, count, alpha=None):

@@ -5,6 +5,7 @@
 #include <stdint.h>
 
 #include <algorithm>
+#include <atomic>
 #include <chrono>
 #include <cmath>
 #include <new>
@@ -55,11 +56,15 @@ struct mf_hip_plan {
   hipStream_t side[NSIDE] = {};
   hipEvent_t fork = nullptr, join[NSIDE] = {};
   int lane_of_cls[NCLS] = {};             // 0: the caller's stream, k: side[k - 1]
+  // sequences captured from this plan and not yet destroyed: their graphs hold the plan's device tables and side
+  // streams, so the plan refuses to be destroyed while any is alive (mutable: sequences take a const plan)
+  mutable std::atomic<int64_t> live_sequences{0};
 };
 
 struct mf_hip_sequence {
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
+  const mf_hip_plan* plan = nullptr;      // (its live_sequences counts this sequence)
 };
 
 namespace {
@@ -281,6 +286,7 @@ extern "C" int mf_hip_plan_create(const mf_hip_group* groups, int64_t ngroups, m
 }
 
 extern "C" int mf_hip_plan_destroy(mf_hip_plan* p) {
+  if (p && p->live_sequences.load() > 0) return -1;     // (a sequence's graph still points at the plan's tables)
   free_plan(p);
   return 0;
 }
@@ -409,6 +415,8 @@ extern "C" int mf_hip_sequence_create(const mf_hip_plan* p, const double* A0, do
     delete s;
     return rc;
   }
+  s->plan = p;
+  p->live_sequences.fetch_add(1);
   *out = s;
   return 0;
 }
@@ -422,6 +430,7 @@ extern "C" int mf_hip_sequence_destroy(mf_hip_sequence* s) {
   if (!s) return 0;
   if (s->exec) (void)hipGraphExecDestroy(s->exec);
   if (s->graph) (void)hipGraphDestroy(s->graph);
+  if (s->plan) s->plan->live_sequences.fetch_sub(1);
   delete s;
   return 0;
 }

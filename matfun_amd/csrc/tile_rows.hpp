@@ -79,6 +79,20 @@ MF_DEV int64_t row_of(const R& r, int j) {
   return REG == R_IN ? r.m(j) : (REG == R_J ? r.jr(j) : r.hr(j));
 }
 
+// a store of the kernels' outputs (a scalar or a 16-byte vector).  MF_NT_STORE 1: non-temporal stores (the outputs
+// are written once and not read back by the kernel)
+#ifndef MF_NT_STORE
+#define MF_NT_STORE 0
+#endif
+template <class T>
+MF_DEV void out_store(T* p, const T& v) {
+#if MF_NT_STORE
+  __builtin_nontemporal_store(v, p);
+#else
+  *p = v;
+#endif
+}
+
 // out[row(j) N + e] = get(j, e) for the count = N nm elements (t = j N + e) of the tile's matrices j = 0 .. nm-1 in
 // region REG, by the wave, for consecutive rows (RunRows: one contiguous block, out + base N + t; rows listed per
 // matrix take for_row_runs with copy_run / copy_run_lut).  With VEC > 1, VEC consecutive elements per lane and one
@@ -101,18 +115,18 @@ MF_DEV void copy_out(OT* __restrict__ out, const R& rows, int count, int lane, G
         eq = wrap ? 0 : eq + 1;
         jq = wrap ? jq + 1 : jq;
       }
-      *reinterpret_cast<vv*>(blk + t) = val;
+      out_store(reinterpret_cast<vv*>(blk + t), val);
       e += VEC * WAVE;
       while (e >= N) { e -= N; j++; }
     }
     for (; t < count; t++) {                     // (the tail: at most one lane, fewer than VEC elements)
-      blk[t] = get(j, e);
+      out_store(blk + t, get(j, e));
       if (++e == N) { e = 0; j++; }
     }
   } else {
     int j = lane / N, e = lane - j * N;
     for (int t = lane; t < count; t += WAVE) {
-      blk[t] = get(j, e);
+      out_store(blk + t, get(j, e));
       e += WAVE;
       while (e >= N) { e -= N; j++; }
     }
@@ -149,7 +163,7 @@ __device__ void copy_run(OT* __restrict__ o, int count, int lane, Get get) {
   typedef OT vv __attribute__((ext_vector_type(V)));
   const int mis = (int)(((uintptr_t)o / sizeof(OT)) % V);
   const int head = (V - mis) % V < count ? (V - mis) % V : count;
-  if (lane < head) o[lane] = get(lane / N, lane % N);
+  if (lane < head) out_store(o + lane, get(lane / N, lane % N));
   int t = head + V * lane, j = t / N, e = t - j * N;
   for (; t + V - 1 < count; t += V * WAVE) {
     vv val;
@@ -161,12 +175,12 @@ __device__ void copy_run(OT* __restrict__ o, int count, int lane, Get get) {
       eq = wrap ? 0 : eq + 1;
       jq = wrap ? jq + 1 : jq;
     }
-    *reinterpret_cast<vv*>(o + t) = val;
+    out_store(reinterpret_cast<vv*>(o + t), val);
     e += V * WAVE;
     while (e >= N) { e -= N; j++; }
   }
   const int t0 = head + (count - head) / V * V;
-  if (lane < count - t0) o[t0 + lane] = get((t0 + lane) / N, (t0 + lane) % N);
+  if (lane < count - t0) out_store(o + t0 + lane, get((t0 + lane) / N, (t0 + lane) % N));
 }
 
 // Output layouts through a lookup table: element e of a matrix's output row is a value staged in LDS (the matrix's
@@ -196,21 +210,21 @@ __device__ void copy_run_lut(OT* __restrict__ o, int count, int lane, const unsi
   typedef OT vv __attribute__((ext_vector_type(V)));
   const int mis = (int)(((uintptr_t)o / sizeof(OT)) % V);
   const int head = (V - mis) % V < count ? (V - mis) % V : count;
-  if (lane < head) o[lane] = lut_value<OT>(st8, lut[lane]);          // (head < V <= N: the first matrix)
+  if (lane < head) out_store(o + lane, lut_value<OT>(st8, lut[lane]));          // (head < V <= N: the first matrix)
   int t = head + V * lane, j = t / N, e = t - j * N;
   const unsigned char* b = st8 + j * S * (int)sizeof(OT);
   for (; t + V - 1 < count; t += V * WAVE) {
     vv val;
 #pragma unroll
     for (int q = 0; q < V; q++) val[q] = lut_value<OT>(b, lut[e + q]);
-    *reinterpret_cast<vv*>(o + t) = val;
+    out_store(reinterpret_cast<vv*>(o + t), val);
     e += V * WAVE;
     while (e >= N) { e -= N; b += S * (int)sizeof(OT); }
   }
   const int t0 = head + (count - head) / V * V;
   if (lane < count - t0) {
     const int tt = t0 + lane, jj = tt / N;
-    o[tt] = lut_value<OT>(st8 + jj * S * (int)sizeof(OT), lut[tt - jj * N]);
+    out_store(o + tt, lut_value<OT>(st8 + jj * S * (int)sizeof(OT), lut[tt - jj * N]));
   }
 }
 
@@ -233,7 +247,7 @@ MF_DEV void copy_out_lut(OT* __restrict__ out, const R& rows, int count, int lan
       vv val;
 #pragma unroll
       for (int q = 0; q < VEC; q++) val[q] = lut_value<OT>(b, lut[e + q]);
-      *reinterpret_cast<vv*>(blk + t) = val;
+      out_store(reinterpret_cast<vv*>(blk + t), val);
       e += VEC * WAVE;
       if constexpr (N >= VEC * WAVE) {           // (at most one matrix boundary per step: selects, no division)
         const bool w = e >= N;
@@ -244,7 +258,7 @@ MF_DEV void copy_out_lut(OT* __restrict__ out, const R& rows, int count, int lan
       }
     }
     for (; t < count; t++) {                     // (the tail: at most one lane, fewer than VEC elements)
-      blk[t] = lut_value<OT>(b, lut[e]);
+      out_store(blk + t, lut_value<OT>(b, lut[e]));
       if (++e == N) { e = 0; b += S * (int)sizeof(OT); }
     }
   } else if constexpr (!R::RUN) {
@@ -266,7 +280,7 @@ template <int VEC, class OT>
 MF_DEV void copy_block(OT* __restrict__ out, const OT* src, int count, int lane) {
   const int mis = (int)(((uintptr_t)out / sizeof(OT)) % VEC);
   const int head = (VEC - mis) % VEC < count ? (VEC - mis) % VEC : count;
-  if (lane < head) out[lane] = src[lane];
+  if (lane < head) out_store(out + lane, src[lane]);
   const int nv = (count - head) / VEC;
   if constexpr (VEC > 1) {
     typedef OT vv __attribute__((ext_vector_type(VEC)));
@@ -274,13 +288,13 @@ MF_DEV void copy_block(OT* __restrict__ out, const OT* src, int count, int lane)
       vv val;
 #pragma unroll
       for (int q = 0; q < VEC; q++) val[q] = src[head + VEC * v + q];
-      *reinterpret_cast<vv*>(out + head + VEC * v) = val;
+      out_store(reinterpret_cast<vv*>(out + head + VEC * v), val);
     }
   } else {
-    for (int v = lane; v < nv; v += WAVE) out[head + v] = src[head + v];
+    for (int v = lane; v < nv; v += WAVE) out_store(out + head + v, src[head + v]);
   }
   const int t0 = head + VEC * nv;
-  if (lane < count - t0) out[t0 + lane] = src[t0 + lane];
+  if (lane < count - t0) out_store(out + t0 + lane, src[t0 + lane]);
 }
 
 // the tile's inputs (NIN doubles per matrix) into LDS, coalesced

@@ -218,3 +218,41 @@ def test_invalid_plans_and_calls():
     empty.eval(None, out=(torch.empty(0), torch.empty(0), torch.empty(0)))
     b.close()
     empty.close()
+
+
+def test_python_checks_reject_bad_tensors_before_launch():
+    """a wrong dtype, a short or strided tensor, or a host tensor never reaches the C ABI"""
+    import torch
+    import matfun_amd
+    b = matfun_amd.Batch([("smf", "log", 2, 10), ("gmf", "pow", 1, 3, 0.5)])
+    A = torch.from_numpy(W.spd(13, seed=613)).cuda()
+    F, J, H = b.outputs()
+    for bad_call in (lambda: b.eval(A.float()), lambda: b.eval(A[:-1]), lambda: b.eval(A.cpu()),
+                     lambda: b.eval(A, alpha=torch.zeros(12, dtype=torch.float64, device="cuda")),
+                     lambda: b.eval(A, out=(F, J[:-1], H)), lambda: b.eval(A, out=(F, J, H), precision="mixed"),
+                     lambda: b.perturb(A.reshape(13, 9).t(), 1e-3, 1, 1), lambda: b.tune(A, out=(F, None, H))):
+        with pytest.raises(ValueError):
+            bad_call()
+    b.close()
+
+
+def test_plan_outlives_its_sequences():
+    """mf_hip_plan_destroy refuses (-1, nothing freed) while a sequence captured from the plan is alive, and
+    Batch.close raises; once the sequence is destroyed the plan goes.  Nothing is launched."""
+    import torch
+    import matfun_amd
+    gs = [("smf", "log", 2, 5), ("gmf", "exp", 1, 7)]
+    b = matfun_amd.Batch(gs)
+    dA0 = torch.from_numpy(inputs([(g[0], g[1], g[2], g[3], 0.0) for g in gs], 614)).cuda()
+    dA = torch.empty_like(dA0)
+    out = b.outputs()
+    seq = b.sequence(dA0, dA, out, 2, 1e-3, 5)
+    lib = matfun_amd.library()
+    assert lib.mf_hip_plan_destroy(b._h) == -1            # (the C ABI itself refuses)
+    with pytest.raises(RuntimeError):
+        b.close()
+    assert b._h                                           # (still usable)
+    seq.close()
+    b.close()
+    assert not b._h
+    assert lib.mf_hip_plan_destroy(None) == 0

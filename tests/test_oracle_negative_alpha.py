@@ -149,3 +149,81 @@ def test_non_finite_exponent_rejected(oracle):
         al = np.array([0.5, bad, 0.5, 0.5])
         assert oracle.smf_matrix_alpha(al, A, 1)[0] == -1
         assert oracle.gmf_alpha(al, A, 1)[0] == -1
+
+
+def test_gmf_alpha_positive_equals_uniform_calls(oracle):
+    """oracle_gmf_eval_alpha at alpha > 0 is the reference's computation: bitwise equal to the uniform-alpha entry
+    point (which is itself bitwise the reference's scalar build, tests/test_oracle_fixtures.py), for exponents drawn
+    per matrix among 0.3, 1.7, 60.5 (60.5: radius 3/alpha inside the Taylor cluster).  A per-matrix error that scaled
+    alpha would pass the inverse identity above (A^(c alpha) = (A^-1)^(c |alpha|)) but not this."""
+    G = W.general(600, seed=50)
+    rng = np.random.default_rng(51)
+    choices = np.array([0.3, 1.7, 60.5])
+    al = choices[rng.integers(0, 3, len(G))]
+    for order in (0, 1, 2):
+        rc, F, J, H = oracle.gmf_alpha(al, G, order)
+        assert rc == 0
+        for a in choices:
+            s = al == a
+            _, Fu, Ju, Hu = oracle.gmf("pow", a, G[s], order)
+            for x, y in ((F, Fu), (J, Ju), (H, Hu)):
+                if y is not None:
+                    assert np.array_equal(x[s], y, equal_nan=True), (a, order)
+
+
+@pytest.mark.skipif(not __import__("oracle.pyoracle", fromlist=["available"]).available("ref"),
+                    reason="reference C library not built (oracle/_ref)")
+def test_gmf_alpha_positive_equals_reference_build():
+    """... and against the compiled reference itself (scalar variant, fresh process for MATFUN_SIMD)"""
+    import os
+    import subprocess
+    import sys
+    code = r"""
+import sys, numpy as np
+sys.path.insert(0, %r)
+from oracle.pyoracle import Checker
+from matfun_amd import workloads as W
+o, r = Checker('oracle'), Checker('ref')
+assert r.lib.smf_simd_name() == b'scalar'
+G = W.general(800, seed=52)
+for a in (0.3, 1.7, 60.5):
+    for order in (0, 1, 2):
+        x, y = o.gmf_alpha(np.full(len(G), a), G, order), r.gmf('pow', a, G, order)
+        for p, q in zip(x[1:], y[1:]):
+            if q is not None:
+                assert np.array_equal(p, q, equal_nan=True), (a, order)
+print('ok')
+""" % os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    env = dict(os.environ, MATFUN_SIMD="scalar", OMP_NUM_THREADS="4")
+    out = subprocess.run([sys.executable, "-c", code], env=env, capture_output=True, text=True, timeout=300)
+    assert out.returncode == 0 and "ok" in out.stdout, out.stderr[-2000:]
+
+
+def inverse_closed_form(A, sym):
+    """F = A^-1, J[E] = -A^-1 E A^-1, H[E1, E2] = A^-1 E1 A^-1 E2 A^-1 + A^-1 E2 A^-1 E1 A^-1 (the derivatives of
+    the inverse; directions symmetrised for smf), in the matrix layout"""
+    n = len(A)
+    B = inv_ld(A).reshape(n, 3, 3)
+    E = directions(sym)
+    J = -np.einsum("nik,ckl,nlj->nijc", B, E, B)
+    BEB = np.einsum("nik,ckl,nlj->ncij", B, E, B)
+    BEBEB = np.einsum("ncik,ekl,nlj->nijce", BEB, E, B)
+    H = BEBEB + np.swapaxes(BEBEB, 3, 4)
+    return B.reshape(n, 9), J.reshape(n, 81), H.reshape(n, 729)
+
+
+def test_alpha_minus_one_derivatives_are_those_of_the_inverse(oracle):
+    """alpha = -1 at order 2 against the closed-form derivatives of the matrix inverse (no evaluation at a positive
+    exponent involved): smf on SPD matrices (symmetric directions), gmf on well-conditioned general matrices"""
+    A = W.spd(200, seed=53, log10_range=(-1.0, 1.0))
+    _, F, J, H = oracle.smf_matrix_alpha(np.full(len(A), -1.0), A, 2)
+    for x, y in zip((F, J, H), inverse_closed_form(A, sym=True)):
+        assert normwise(x, y).max() <= 1e-13             # (measured 2.8e-15 / 5.4e-15 / 8.3e-15)
+    G = W.general(400, seed=54)
+    ok = np.array([np.linalg.cond(g.reshape(3, 3)) < 50 for g in G])
+    G = G[ok][:150]
+    _, F, J, H = oracle.gmf_alpha(np.full(len(G), -1.0), G, 2)
+    good = ~np.isnan(F).any(axis=1)
+    assert good.sum() > 100
+    for x, y in zip((F, J, H), inverse_closed_form(G, sym=False)):
+        assert normwise(x[good], y[good]).max() <= 1e-13  # (measured 1.2e-15 / 2.3e-15 / 1.1e-14)
