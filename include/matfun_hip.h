@@ -20,6 +20,7 @@
 #ifndef MATFUN_HIP_H
 #define MATFUN_HIP_H
 
+#include <stddef.h>
 #include <stdint.h>
 
 #ifdef __cplusplus
@@ -67,6 +68,27 @@ int smf_hip_eval_matrix_pow_mixed(const double *alpha, int64_t n, const double *
                                   void *stream);
 int gmf_hip_eval_pow_mixed(const double *alpha, int64_t n, const double *A, float *F, float *J, float *H,
                            void *stream);
+
+/* ---- general matrices ordered by series length (no counterpart in the reference API): the results of gmf_hip_eval /
+ * gmf_hip_eval_pow (and their _mixed forms) bitwise, faster on batches whose matrices need group series of very
+ * different lengths.  A wavefront of 64 matrices runs its series loop as long as its longest series (on the general
+ * workload mix the longest of 64 averages 3x the mean), so the matrices are first sorted by series length on the
+ * device (a length pass, a counting sort into a permutation) and then evaluated in that order; every matrix still
+ * gets its own results written to its own rows.  `work` is caller-owned device memory of work_bytes bytes,
+ * gmf_hip_sort_workspace(n) for the whole batch in one pass (about 5 bytes per matrix plus 200 KB); a smaller
+ * workspace runs the batch in consecutive parts of what it holds.  Same arguments, layouts, return codes and
+ * asynchrony as gmf_hip_eval, plus -1 when n > 0 and the workspace is NULL or holds fewer than min(n, 64) matrices
+ * (gmf_hip_sort_workspace(n) bytes always suffice).  Calls
+ * that share a workspace must not overlap (stream order suffices). */
+size_t gmf_hip_sort_workspace(int64_t n);
+int gmf_hip_eval_sorted(int kind, double alpha, int64_t n, const double *A, double *F, double *J, double *H, void *work,
+                        size_t work_bytes, void *stream);
+int gmf_hip_eval_sorted_mixed(int kind, double alpha, int64_t n, const double *A, float *F, float *J, float *H,
+                              void *work, size_t work_bytes, void *stream);
+int gmf_hip_eval_pow_sorted(const double *alpha, int64_t n, const double *A, double *F, double *J, double *H,
+                            void *work, size_t work_bytes, void *stream);
+int gmf_hip_eval_pow_sorted_mixed(const double *alpha, int64_t n, const double *A, float *F, float *J, float *H,
+                                  void *work, size_t work_bytes, void *stream);
 
 /* ---- packed symmetric outputs (no counterpart in the reference API; the H packing is the reference kernel's own
  * HPACK layout, c/include/matfun/smf_kernel.hpp:196-199): Voigt input v (n,6) as smf_hip_eval, outputs of the

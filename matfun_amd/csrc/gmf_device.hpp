@@ -272,8 +272,16 @@ MF_DEV void gmf_series(const GmfSeriesIn& s, const double* __restrict__ bt, cpx*
 // the series computed by the lane itself
 template <int KIND, int ORDER, bool PA>
 struct GmfSeriesLocal {
+  static constexpr bool K_ONLY = false;
   const double* bt;
   MF_DEV void operator()(const GmfSeriesIn& s, cpx* acc) const { gmf_series<KIND, ORDER, PA>(s, bt, acc); }
+};
+
+// no series: gmf_lane stops once the series length is known and returns it in GmfOut::K (the first pass of the
+// length-sorted evaluation, gmf_sorted.hip)
+struct GmfSeriesLength {
+  static constexpr bool K_ONLY = true;
+  MF_DEV void operator()(const GmfSeriesIn&, cpx*) const {}
 };
 
 // results of one matrix needed for the output assembly
@@ -283,6 +291,7 @@ struct GmfOut {
   double D2[9];       // Re(T^T D T) on two indices (order >= 1)
   double D3[27];      // ... on three indices (order >= 2)
   double sc1, sc2;    // factors of J and H (NaN for a matrix outside the domain)
+  int K;              // series length (set only by a GmfSeriesLength pass)
 };
 
 // PA (powers with a per-matrix exponent al): bt holds 1 / j instead of the coefficients b_j (gmf_recips)
@@ -523,6 +532,10 @@ MF_DEV void gmf_lane(const double* a, double al, const double* __restrict__ bt, 
   {
     GmfSeriesIn sin;
     sin.K = hasg ? series_terms<PA>(bt, al, 3 * (ORDER + 1) - 1, rr, rq, rinv) : 0;
+    if constexpr (SER::K_ONLY) {
+      o.K = sin.K;
+      return;
+    }
     sin.b0 = (hasg && KIND == LOG) ? node_value<LOG>(al, sig, m, mk(cxg, 0.0), mk(cmu, 0.0), 0).re : 1.0;
 #pragma unroll
     for (int i = 0; i < 3; i++) sin.u[i] = u[i];

@@ -1,0 +1,239 @@
+// gmf_sorted.hip -- the general-matrix evaluation with the matrices ordered by series length.
+//
+// A wavefront runs its group-series loop to the longest series among its 64 lanes, and the lengths spread widely
+// (general mix, log order 2: mean 51 terms, median 43, while a wavefront's longest averages 165), so most lanes idle
+// through most of the loop.  Here the batch is evaluated in three steps on the stream:
+//   1. gmf_klen_kernel   one lane per matrix runs the per-matrix math up to the series length K (gmf_lane with
+//                        GmfSeriesLength) and stores the key 200 - K (one byte; longest first);
+//   2. a counting sort of the keys (key_hist_kernel: a histogram per chunk of the batch; key_scan_kernel: every
+//                        chunk's offset per key; key_scatter_kernel: each matrix's position) into a permutation;
+//   3. gmf_perm_kernel   the tile body of the homogeneous kernel (gmf_tile) over tiles of 64 consecutive entries of
+//                        the permutation (PermRows): every matrix is computed by the same code as in gmf_hip_eval,
+//                        so its results are those of gmf_hip_eval bitwise; only the grouping of matrices into
+//                        wavefronts changes, and with it how long each wavefront's series loop runs.
+// Workspace (caller-owned device memory, gmf_sort_workspace): keys (1 B per matrix), the permutation (4 B per matrix),
+// two tables of KSORT_MAXCH x KBINS counts.  A batch larger than the workspace holds runs in consecutive parts.
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "gmf_sorted.hpp"
+#include "gmf_tile.hpp"
+
+namespace mf {
+
+namespace {
+
+constexpr int KLEN_THREADS = 256, SORT_THREADS = 1024;
+
+size_t align256(size_t b) { return (b + 255) / 256 * 256; }
+
+// the keys 200 - K of matrices [0, n) (one lane per matrix)
+template <int KIND, int ORDER, bool PA>
+__global__ void __launch_bounds__(KLEN_THREADS)
+    gmf_klen_kernel(int64_t n, const double* __restrict__ A, double al, const double* __restrict__ alv, GmfCoef bt,
+                    uint8_t* __restrict__ key) {
+  const int64_t i = (int64_t)blockIdx.x * KLEN_THREADS + threadIdx.x;
+  if (i >= n) return;
+  double a[9];
+#pragma unroll
+  for (int k = 0; k < 9; k++) a[k] = A[9 * i + k];
+  if (PA) {                                        // (as gmf_tile: a non-finite exponent computes with 1)
+    al = alv[i];
+    al = fabs(al) <= 1.7976931348623157e308 ? al : 1.0;
+  }
+  GmfOut o;
+  gmf_lane<KIND, ORDER, PA>(a, al, bt.b, o, g_gmf_recip.r, GmfSeriesLength{});
+  key[i] = (uint8_t)(GMF_KMAX - o.K);
+}
+
+// hist[c][b]: how many of the keys in chunk c (matrices [c chunk, (c + 1) chunk)) equal b
+__global__ void __launch_bounds__(SORT_THREADS)
+    key_hist_kernel(int64_t n, int64_t chunk, const uint8_t* __restrict__ key, uint32_t* __restrict__ hist) {
+  __shared__ uint32_t h[KBINS];
+  for (int b = threadIdx.x; b < KBINS; b += SORT_THREADS) h[b] = 0;
+  __syncthreads();
+  const int64_t lo = (int64_t)blockIdx.x * chunk, hi = lo + chunk < n ? lo + chunk : n;
+  for (int64_t i = lo + threadIdx.x; i < hi; i += SORT_THREADS) atomicAdd(&h[key[i]], 1u);
+  __syncthreads();
+  for (int b = threadIdx.x; b < KBINS; b += SORT_THREADS) hist[(int64_t)blockIdx.x * KBINS + b] = h[b];
+}
+
+// off[c][b] = (matrices with a key below b) + (matrices with key b in the chunks before c): where chunk c's matrices
+// of key b start in the permutation.  One workgroup; the nch x KBINS table is scanned in LDS.
+__global__ void __launch_bounds__(SORT_THREADS)
+    key_scan_kernel(int nch, const uint32_t* __restrict__ hist, uint32_t* __restrict__ off) {
+  __shared__ uint32_t t[KSORT_MAXCH * KBINS];
+  __shared__ uint32_t base[KBINS];
+  const int cells = nch * KBINS;
+  for (int k = threadIdx.x; k < cells; k += SORT_THREADS) t[k] = hist[k];
+  __syncthreads();
+  if (threadIdx.x < KBINS) {                       // per key: running sum over the chunks (exclusive)
+    const int b = threadIdx.x;
+    uint32_t s = 0;
+    for (int c = 0; c < nch; c++) {
+      const uint32_t v = t[c * KBINS + b];
+      t[c * KBINS + b] = s;
+      s += v;
+    }
+    base[b] = s;                                   // (the key's total)
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {                          // exclusive scan of the totals over the keys
+    uint32_t s = 0;
+    for (int b = 0; b < KBINS; b++) {
+      const uint32_t v = base[b];
+      base[b] = s;
+      s += v;
+    }
+  }
+  __syncthreads();
+  for (int k = threadIdx.x; k < cells; k += SORT_THREADS) off[k] = t[k] + base[k % KBINS];
+}
+
+// perm[position] = matrix, positions of chunk c's matrices of key b from off[c][b] on (their order within a chunk
+// and key is that of the LDS atomics: any order gives the same results)
+__global__ void __launch_bounds__(SORT_THREADS)
+    key_scatter_kernel(int64_t n, int64_t chunk, const uint8_t* __restrict__ key, const uint32_t* __restrict__ off,
+                       int32_t* __restrict__ perm) {
+  __shared__ uint32_t cur[KBINS];
+  for (int b = threadIdx.x; b < KBINS; b += SORT_THREADS) cur[b] = off[(int64_t)blockIdx.x * KBINS + b];
+  __syncthreads();
+  const int64_t lo = (int64_t)blockIdx.x * chunk, hi = lo + chunk < n ? lo + chunk : n;
+  for (int64_t i = lo + threadIdx.x; i < hi; i += SORT_THREADS) {
+    const uint32_t pos = atomicAdd(&cur[key[i]], 1u);
+    perm[pos] = (int32_t)i;
+  }
+}
+
+// the homogeneous kernel's tile body over perm[64 t .. 64 t + 63]; register budgets as gmf_kernel's one-tile
+// workgroups (order 2 with per-matrix exponents: two wavefronts per SIMD).  Workgroup b takes tile t = b step mod T
+// (T tiles, step coprime to T, about T / SORT_MIX): the workgroups resident at one time hold tiles from every part
+// of the length order -- long series (compute) next to short ones (stores) -- instead of the longest ones together
+// and then the shortest (measured on MI355X against the plain order: 1-6% faster at order 2; SORT_MIX 16 or 64
+// gained less)
+constexpr int64_t SORT_MIX = 4;
+template <int KIND, int ORDER, class OT, bool PA>
+__global__ void __launch_bounds__(WAVE) __attribute__((amdgpu_waves_per_eu((ORDER == 2 && PA) ? 2 : 1)))
+    gmf_perm_kernel(int64_t n, int64_t step, const int32_t* __restrict__ perm, const double* __restrict__ A,
+                    OT* __restrict__ F, OT* __restrict__ J, OT* __restrict__ H, double al,
+                    const double* __restrict__ alv, GmfCoef bt) {
+  __shared__ __attribute__((aligned(16))) unsigned char lds[GmfLds<ORDER, OT>::BYTES];
+  __shared__ int32_t rows[WAVE];
+  const int lane = threadIdx.x;
+  const int64_t T = (n + WAVE - 1) / WAVE, t = (int64_t)blockIdx.x * step % T;
+  const int64_t base = t * WAVE, left = n - base;
+  const int nm = left < WAVE ? (int)left : WAVE;
+  const bool in = lane < nm;
+  const int32_t r = in ? perm[base + lane] : 0;
+  rows[lane] = r;
+  const int32_t rp = __shfl_up(r, 1);
+  const uint64_t cont = __ballot(in && lane > 0 && r == rp + 1);
+  __syncthreads();
+  gmf_tile<KIND, ORDER, OT, PA, PermRows>(PermRows{rows, {cont, cont, cont}}, nm, A, F, J, H, al, alv, bt.b, lds,
+                                          lane);
+}
+
+template <int KIND, bool PA>
+void klen_launch(hipStream_t st, int order, int64_t n, const double* A, double al, const double* alv,
+                 const GmfCoef& bt, uint8_t* key) {
+  const unsigned g = (unsigned)((n + KLEN_THREADS - 1) / KLEN_THREADS);
+  if (order == 2)
+    hipLaunchKernelGGL((gmf_klen_kernel<KIND, 2, PA>), dim3(g), dim3(KLEN_THREADS), 0, st, n, A, al, alv, bt, key);
+  else if (order == 1)
+    hipLaunchKernelGGL((gmf_klen_kernel<KIND, 1, PA>), dim3(g), dim3(KLEN_THREADS), 0, st, n, A, al, alv, bt, key);
+  else
+    hipLaunchKernelGGL((gmf_klen_kernel<KIND, 0, PA>), dim3(g), dim3(KLEN_THREADS), 0, st, n, A, al, alv, bt, key);
+}
+
+int64_t gcd64(int64_t a, int64_t b) { return b ? gcd64(b, a % b) : a; }
+
+template <int KIND, class OT, bool PA>
+void perm_launch(hipStream_t st, int64_t n, const int32_t* perm, const double* A, OT* F, OT* J, OT* H, double al,
+                 const double* alv, const GmfCoef& bt) {
+  const int64_t T = (n + WAVE - 1) / WAVE;
+  int64_t step = T / SORT_MIX > 1 ? T / SORT_MIX : 1;
+  while (gcd64(step, T) != 1) step++;                     // (a bijection of the tiles)
+  const unsigned g = (unsigned)T;
+  if (H)
+    hipLaunchKernelGGL((gmf_perm_kernel<KIND, 2, OT, PA>), dim3(g), dim3(WAVE), 0, st, n, step, perm, A, F, J, H, al,
+                       alv, bt);
+  else if (J)
+    hipLaunchKernelGGL((gmf_perm_kernel<KIND, 1, OT, PA>), dim3(g), dim3(WAVE), 0, st, n, step, perm, A, F, J, H, al,
+                       alv, bt);
+  else
+    hipLaunchKernelGGL((gmf_perm_kernel<KIND, 0, OT, PA>), dim3(g), dim3(WAVE), 0, st, n, step, perm, A, F, J, H, al,
+                       alv, bt);
+}
+
+// the workspace of a part of m matrices
+struct SortSpace {
+  uint8_t* key;
+  int32_t* perm;
+  uint32_t *hist, *off;
+};
+SortSpace carve(void* work, int64_t m) {
+  unsigned char* p = static_cast<unsigned char*>(work);
+  SortSpace s;
+  s.key = p;
+  p += align256((size_t)m);
+  s.perm = reinterpret_cast<int32_t*>(p);
+  p += align256(4 * (size_t)m);
+  s.hist = reinterpret_cast<uint32_t*>(p);
+  p += align256(4 * (size_t)KSORT_MAXCH * KBINS);
+  s.off = reinterpret_cast<uint32_t*>(p);
+  return s;
+}
+
+// one part of m <= capacity matrices: keys, sort, evaluation
+template <int KIND, class OT, bool PA>
+void sorted_part(hipStream_t st, int order, int64_t m, const double* A, OT* F, OT* J, OT* H, double al,
+                 const double* alv, const GmfCoef& bt, void* work) {
+  const SortSpace s = carve(work, m);
+  // chunks of the histogram: at most KSORT_MAXCH, each a multiple of the workgroup size
+  int64_t chunk = (m + KSORT_MAXCH - 1) / KSORT_MAXCH;
+  chunk = (chunk + SORT_THREADS - 1) / SORT_THREADS * SORT_THREADS;
+  const int nch = (int)((m + chunk - 1) / chunk);
+  klen_launch<KIND, PA>(st, order, m, A, al, alv, bt, s.key);
+  hipLaunchKernelGGL(key_hist_kernel, dim3(nch), dim3(SORT_THREADS), 0, st, m, chunk, s.key, s.hist);
+  hipLaunchKernelGGL(key_scan_kernel, dim3(1), dim3(SORT_THREADS), 0, st, nch, s.hist, s.off);
+  hipLaunchKernelGGL(key_scatter_kernel, dim3(nch), dim3(SORT_THREADS), 0, st, m, chunk, s.key, s.off, s.perm);
+  perm_launch<KIND, OT, PA>(st, m, s.perm, A, F, J, H, al, alv, bt);
+}
+
+}  // namespace
+
+size_t gmf_sort_workspace(int64_t n) {
+  if (n < 0) return 0;
+  return align256((size_t)n) + align256(4 * (size_t)n) + 2 * align256(4 * (size_t)KSORT_MAXCH * KBINS);
+}
+
+int64_t gmf_sort_capacity(size_t bytes) {
+  const size_t fixed = 2 * align256(4 * (size_t)KSORT_MAXCH * KBINS);
+  if (bytes < fixed) return 0;
+  int64_t m = (int64_t)((bytes - fixed) / 5);
+  while (m > 0 && gmf_sort_workspace(m) > bytes) m--;       // (at most ~100 steps: the alignment padding)
+  return m;
+}
+
+template <class OT>
+int gmf_sorted_launch(hipStream_t st, int kind, int order, int64_t n, const double* A, OT* F, OT* J, OT* H, double al,
+                      const double* alv, const GmfCoef& bt, void* work, int64_t cap) {
+  for (int64_t lo = 0; lo < n; lo += cap) {
+    const int64_t m = n - lo < cap ? n - lo : cap;
+    const double* Ap = A + 9 * lo;
+    OT *Fp = F + 9 * lo, *Jp = J ? J + 81 * lo : nullptr, *Hp = H ? H + 729 * lo : nullptr;
+    const double* ap = alv ? alv + lo : nullptr;
+    if (alv) sorted_part<POW, OT, true>(st, order, m, Ap, Fp, Jp, Hp, 1.0, ap, bt, work);
+    else if (kind == EXP) sorted_part<EXP, OT, false>(st, order, m, Ap, Fp, Jp, Hp, al, nullptr, bt, work);
+    else if (kind == LOG) sorted_part<LOG, OT, false>(st, order, m, Ap, Fp, Jp, Hp, al, nullptr, bt, work);
+    else sorted_part<POW, OT, false>(st, order, m, Ap, Fp, Jp, Hp, al, nullptr, bt, work);
+    if (hipGetLastError() != hipSuccess) return -2;
+  }
+  return 0;
+}
+template int gmf_sorted_launch<double>(hipStream_t, int, int, int64_t, const double*, double*, double*, double*,
+                                       double, const double*, const GmfCoef&, void*, int64_t);
+template int gmf_sorted_launch<float>(hipStream_t, int, int, int64_t, const double*, float*, float*, float*, double,
+                                      const double*, const GmfCoef&, void*, int64_t);
+
+}  // namespace mf

@@ -36,6 +36,7 @@ struct GmfSeriesShared {
   const double* bt;
   unsigned char* pool;            // LDS, 16-byte aligned
   int tid;                        // threadIdx.x
+  static constexpr bool K_ONLY = false;
   // every lane of the workgroup calls it once per tile (five workgroup barriers)
   __device__ void operator()(const GmfSeriesIn& s, cpx* acc) const {
     using P = GmfPool<ORDER, NW>;
@@ -156,6 +157,8 @@ MF_DEV void gmf_tile(const R& rows, int nm, const double* __restrict__ A, OT* __
   wave_sync();
   if constexpr (R::RUN) {
     copy_block<VEC>(F + rows.m(0) * 9, xr, 9 * nm, lane);
+  } else if constexpr (R::SCATTER) {
+    copy_rows_scattered<9, R_IN>(F, rows, xr, 9 * nm, lane);
   } else {                                           // (rows listed: runs of consecutive rows as blocks)
     for_row_runs<R_IN>(rows, nm, lane,
                        [&](int j0, int j1) { copy_block<VEC>(F + rows.m(j0) * 9, xr + 9 * j0, 9 * (j1 - j0), lane); });
@@ -203,6 +206,8 @@ MF_DEV void gmf_tile(const R& rows, int nm, const double* __restrict__ A, OT* __
       wave_sync();
       if constexpr (R::RUN) {
         copy_block<VEC>(J + rows.jr(p0) * 81, xr, 81 * cnt, lane);
+      } else if constexpr (R::SCATTER) {
+        copy_rows_scattered<81, R_J>(J, shift_rows(rows, p0), xr, 81 * cnt, lane);
       } else {
         const R sub = shift_rows(rows, p0);
         for_row_runs<R_J>(sub, cnt, lane, [&](int j0, int j1) {

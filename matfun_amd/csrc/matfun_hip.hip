@@ -6,6 +6,7 @@
 
 #include "../../include/matfun_hip.h"
 #include "gmf_device.hpp"
+#include "gmf_sorted.hpp"
 #include "smf_device.hpp"
 #include "smf_launch.hpp"
 
@@ -97,6 +98,29 @@ int gmf_call(int kind, double alpha, int64_t n, const double* A, OT* F, OT* J, O
     if (launched()) return -2;
   }
   return 0;
+}
+
+// general matrices ordered by series length through the caller's workspace (gmf_sorted.hip); the same checks and
+// normalisation as gmf_call / gmf_pow_call (alv: per-matrix exponents, else kind and alpha), plus -1 for a workspace
+// that holds fewer than min(n, 64) matrices
+constexpr int64_t SORT_PART_MAX = (int64_t)1 << 30;      // matrices per part (32-bit permutation entries)
+template <class OT>
+int gmf_sorted_call(int kind, double alpha, const double* alv, int64_t n, const double* A, OT* F, OT* J, OT* H,
+                    void* work, size_t work_bytes, void* stream) {
+  if (alv) kind = mf::POW, alpha = 1.0;
+  if (!valid(kind, alpha, n, J, H)) return -1;
+  if (kind == mf::SQRT) kind = mf::POW, alpha = 0.5;
+  if (kind != mf::POW) alpha = 0.0;
+  if (n == 0) return 0;
+  int64_t cap = mf::gmf_sort_capacity(work ? work_bytes : 0);
+  if (cap < (n < 64 ? n : 64)) return -1;
+  if (n > GMF_MAX_N) return -2;
+  cap = cap < SORT_PART_MAX ? cap : SORT_PART_MAX;
+  mf::GmfCoef bt;
+  if (alv) mf::gmf_recips(bt);
+  else mf::gmf_coefs(kind, alpha, bt);
+  const int order = H ? 2 : (J ? 1 : 0);
+  return mf::gmf_sorted_launch<OT>((hipStream_t)stream, kind, order, n, A, F, J, H, alpha, alv, bt, work, cap);
 }
 
 void packed_chunk(hipStream_t st, int kind, int64_t n, const double* x, double* F, double* J, double* H, double al,
@@ -228,6 +252,30 @@ extern "C" int smf_hip_eval_packed(int kind, double alpha, int64_t n, const doub
 extern "C" int smf_hip_eval_packed_mixed(int kind, double alpha, int64_t n, const double* v, float* F, float* J,
                                          float* H, void* stream) {
   return smf_packed_call(kind, alpha, n, v, F, J, H, stream);
+}
+
+extern "C" size_t gmf_hip_sort_workspace(int64_t n) { return mf::gmf_sort_workspace(n); }
+
+extern "C" int gmf_hip_eval_sorted(int kind, double alpha, int64_t n, const double* A, double* F, double* J, double* H,
+                                   void* work, size_t work_bytes, void* stream) {
+  return gmf_sorted_call(kind, alpha, nullptr, n, A, F, J, H, work, work_bytes, stream);
+}
+
+extern "C" int gmf_hip_eval_sorted_mixed(int kind, double alpha, int64_t n, const double* A, float* F, float* J,
+                                         float* H, void* work, size_t work_bytes, void* stream) {
+  return gmf_sorted_call(kind, alpha, nullptr, n, A, F, J, H, work, work_bytes, stream);
+}
+
+extern "C" int gmf_hip_eval_pow_sorted(const double* alpha, int64_t n, const double* A, double* F, double* J,
+                                       double* H, void* work, size_t work_bytes, void* stream) {
+  if (n > 0 && !alpha) return -1;
+  return gmf_sorted_call(mf::POW, 1.0, n > 0 ? alpha : nullptr, n, A, F, J, H, work, work_bytes, stream);
+}
+
+extern "C" int gmf_hip_eval_pow_sorted_mixed(const double* alpha, int64_t n, const double* A, float* F, float* J,
+                                             float* H, void* work, size_t work_bytes, void* stream) {
+  if (n > 0 && !alpha) return -1;
+  return gmf_sorted_call(mf::POW, 1.0, n > 0 ? alpha : nullptr, n, A, F, J, H, work, work_bytes, stream);
 }
 
 extern "C" int smf_hip_set_tile(int kind, int matrix_layout, int mixed, int hpp) {

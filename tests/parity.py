@@ -30,3 +30,25 @@ def maxrel(x, ref):
         return np.inf
     s = np.abs(ref[~both]).max() if (~both).any() else 0.0
     return d.max() / s if d.size else 0.0
+
+
+def spread_check(name, got, ora, refv, tol=1e-12, factor=32.0):
+    """per-matrix normwise error of `got` against the oracle, bounded per matrix by max(tol, factor s) with s the
+    difference between two correct builds of the reference on that matrix -- its SIMD build (libmvec; refv) against
+    its scalar build (= the oracle).  Without the reference build (refv None): p99.9 <= 2 tol and max <= 1e-10.
+    Prints the distributions (max, p99.9, share of matrices above tol) of both as a line starting with "DIST"."""
+    e = normwise(got, ora)
+    line = (f"DIST {name}: n {len(e)} max {e.max():.2e} p99.9 {np.quantile(e, 0.999):.2e} "
+            f"above {tol:g} {np.mean(e > tol):.3%}")
+    if refv is None:
+        print(line + " | reference SIMD build not available", flush=True)
+        assert np.quantile(e, 0.999) <= 2 * tol and e.max() <= 1e-10, (name, e.max(), np.quantile(e, 0.999))
+        return e
+    s = normwise(refv, ora)
+    print(line + f" | reference SIMD vs scalar: max {s.max():.2e} p99.9 {np.quantile(s, 0.999):.2e} "
+                 f"above {tol:g} {np.mean(s > tol):.3%}", flush=True)
+    bound = np.maximum(tol, factor * s)
+    bad = np.nonzero(e > bound)[0]
+    assert bad.size == 0, (f"{name}: {bad.size} matrices over tolerance, first {bad[0]}: {e[bad[0]]:.2e} "
+                           f"(spread {s[bad[0]]:.2e})")
+    return e

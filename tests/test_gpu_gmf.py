@@ -16,7 +16,7 @@ import pytest
 
 from oracle.pyoracle import Checker, available
 from matfun_amd import workloads as W
-from tests.parity import maxrel, normwise
+from tests.parity import maxrel, normwise, spread_check
 
 pytestmark = pytest.mark.gpu
 GOLD = os.path.join(os.path.dirname(__file__), "golden")
@@ -43,14 +43,7 @@ def gen():
 
 
 def check_spread(name, got, ora, refv):
-    e = normwise(got, ora)
-    if refv is None:
-        assert np.quantile(e, 0.999) <= 2e-12 and e.max() <= 1e-10, (name, e.max(), np.quantile(e, 0.999))
-        return
-    s = normwise(refv, ora)
-    tol = np.maximum(TOL, 32 * s)
-    bad = np.nonzero(e > tol)[0]
-    assert bad.size == 0, f"{name}: {bad.size} matrices over tolerance, first {bad[0]}: {e[bad[0]]:.2e} (spread {s[bad[0]]:.2e})"
+    spread_check(name, got, ora, refv, tol=TOL)
 
 
 @pytest.mark.parametrize("kind,alpha", KINDS)

@@ -1,0 +1,170 @@
+"""gmf_hip_eval_sorted* (general matrices evaluated in the order of their series lengths, an extension without
+counterpart in the reference API) on the GPU.  Every matrix runs the same tile code as in gmf_hip_eval, only grouped
+with other matrices of similar series length, so the contract is bitwise equality with gmf_hip_eval /
+gmf_hip_eval_pow (which the other GPU tests pin to the oracle), for every kind, order and precision, at tile edges,
+with workspaces that hold only part of the batch, and with NaN results only for the matrices outside the domain."""
+import numpy as np
+import pytest
+
+from matfun_amd import workloads as W
+
+pytestmark = pytest.mark.gpu
+KINDS = [("exp", 0.0), ("log", 0.0), ("sqrt", 0.0), ("pow", 0.3), ("pow", 1.7)]
+
+
+def run(X, kind, alpha, order, mixed, sorted_, work_bytes=None, alphas=None):
+    """one call on host array X (n, 9); returns F, J, H as NumPy (J / H None below the order)"""
+    import torch
+    import matfun_amd
+    lib = matfun_amd.library()
+    dt = torch.float32 if mixed else torch.float64
+    n = len(X)
+    dX = torch.from_numpy(np.ascontiguousarray(X)).cuda()
+    outs = [torch.full((n, k), 7.0, dtype=dt, device="cuda") if order >= o else None
+            for o, k in ((0, 9), (1, 81), (2, 729))]
+    p = lambda t: None if t is None else t.data_ptr()
+    st = torch.cuda.current_stream().cuda_stream
+    dal = torch.from_numpy(np.ascontiguousarray(alphas)).cuda() if alphas is not None else None
+    if sorted_:
+        nb = lib.gmf_hip_sort_workspace(n) if work_bytes is None else work_bytes
+        work = torch.empty(max(nb, 1), dtype=torch.uint8, device="cuda")
+        if dal is not None:
+            fn = lib.gmf_hip_eval_pow_sorted_mixed if mixed else lib.gmf_hip_eval_pow_sorted
+            rc = fn(p(dal), n, p(dX), *map(p, outs), work.data_ptr(), nb, st)
+        else:
+            fn = lib.gmf_hip_eval_sorted_mixed if mixed else lib.gmf_hip_eval_sorted
+            rc = fn(matfun_amd.KINDS[kind], alpha, n, p(dX), *map(p, outs), work.data_ptr(), nb, st)
+    else:
+        if dal is not None:
+            fn = lib.gmf_hip_eval_pow_mixed if mixed else lib.gmf_hip_eval_pow
+            rc = fn(p(dal), n, p(dX), *map(p, outs), st)
+        else:
+            fn = lib.gmf_hip_eval_mixed if mixed else lib.gmf_hip_eval
+            rc = fn(matfun_amd.KINDS[kind], alpha, n, p(dX), *map(p, outs), st)
+    torch.cuda.synchronize()
+    assert rc == 0, rc
+    return [None if t is None else t.cpu().numpy() for t in outs]
+
+
+def same(a, b):
+    for x, y in zip(a, b):
+        if y is not None:
+            assert np.array_equal(x, y, equal_nan=True)
+
+
+@pytest.fixture(scope="module")
+def gen():
+    return W.general(3000, seed=701)
+
+
+@pytest.mark.parametrize("mixed", [False, True])
+@pytest.mark.parametrize("order", [0, 1, 2])
+@pytest.mark.parametrize("kind,alpha", KINDS)
+def test_sorted_equals_gmf_hip_eval(gen, kind, alpha, order, mixed):
+    print(f"sorted vs plain: {kind} {alpha} order {order} mixed {mixed}", flush=True)
+    same(run(gen, kind, alpha, order, mixed, True), run(gen, kind, alpha, order, mixed, False))
+
+
+@pytest.mark.parametrize("n", [1, 63, 64, 65, 129, 1000])
+def test_tile_edges(gen, n):
+    X = gen[:n]
+    for kind, alpha in (("log", 0.0), ("pow", 0.3)):
+        same(run(X, kind, alpha, 2, False, True), run(X, kind, alpha, 2, False, False))
+
+
+@pytest.mark.parametrize("mixed", [False, True])
+@pytest.mark.parametrize("order", [1, 2])
+def test_per_matrix_exponents(gen, order, mixed):
+    al = np.random.default_rng(702).uniform(-1.0, 1.0, len(gen))
+    al[:4] = [0.0, 1.0, -1.0, 0.5]
+    same(run(gen, "pow", 0.0, order, mixed, True, alphas=al), run(gen, "pow", 0.0, order, mixed, False, alphas=al))
+
+
+def test_workspace_holding_part_of_the_batch(gen):
+    """a workspace for 64, 100 or 1000 matrices runs the 3000 in parts: the same results"""
+    import matfun_amd
+    lib = matfun_amd.library()
+    ref = run(gen, "log", 0.0, 2, False, False)
+    for m in (64, 100, 1000):
+        nb = lib.gmf_hip_sort_workspace(m)
+        same(run(gen, "log", 0.0, 2, False, True, work_bytes=nb), ref)
+
+
+def test_nan_only_outside_the_domain():
+    """real eigenvalues <= 0 give NaN for log / sqrt / pow in their matrix only, wherever the sort puts it"""
+    A = 1.5 * np.eye(3) + 0.3 * np.random.default_rng(703).standard_normal((200, 3, 3))
+    A[[4, 77, 150]] = np.diag([-1.0, 2.0, 3.0])
+    A = A.reshape(-1, 9)
+    for kind in ("log", "sqrt"):
+        F, J, H = run(A, kind, 0.0, 2, False, True)
+        bad = np.isnan(F).any(axis=1)
+        assert np.nonzero(bad)[0].tolist() == [4, 77, 150]
+        same((F, J, H), run(A, kind, 0.0, 2, False, False))
+
+
+def test_rejections():
+    import torch
+    import matfun_amd
+    lib = matfun_amd.library()
+    X = torch.from_numpy(W.general(100, seed=704)).cuda()
+    F = torch.empty((100, 9), dtype=torch.float64, device="cuda")
+    small = lib.gmf_hip_sort_workspace(64) - 1                  # (holds fewer than 64 matrices)
+    w = torch.empty(lib.gmf_hip_sort_workspace(100), dtype=torch.uint8, device="cuda")
+    st = torch.cuda.current_stream().cuda_stream
+    assert lib.gmf_hip_eval_sorted(1, 0.0, 100, X.data_ptr(), F.data_ptr(), None, None, None, 0, st) == -1
+    assert lib.gmf_hip_eval_sorted(1, 0.0, 100, X.data_ptr(), F.data_ptr(), None, None, w.data_ptr(), small, st) == -1
+    assert lib.gmf_hip_eval_sorted(2, -1.0, 100, X.data_ptr(), F.data_ptr(), None, None, w.data_ptr(), w.numel(),
+                                   st) == -1
+    assert lib.gmf_hip_eval_pow_sorted(None, 100, X.data_ptr(), F.data_ptr(), None, None, w.data_ptr(), w.numel(),
+                                       st) == -1
+    assert lib.gmf_hip_eval_sorted(1, 0.0, 0, None, None, None, None, None, 0, st) == 0
+
+
+def test_python_front_end_sort_flag(gen):
+    import matfun_amd
+    for prec in ("fp64", "mixed"):
+        a = matfun_amd.gmf("log", gen.reshape(-1, 3, 3), order=2, precision=prec, sort=True)
+        b = matfun_amd.gmf("log", gen.reshape(-1, 3, 3), order=2, precision=prec)
+        same(a, b)
+
+
+def test_full_size_general_workload_against_the_oracle():
+    """the general-matrix configuration at full size, n = 2^20, log and pow(0.3) at order 2 through the sorted path:
+    4096 rows spread over the batch plus the last 64 checked against gmf_hip_eval on the same rows (bitwise) and
+    against the oracle (tests.parity.spread_check: per matrix max(1e-12, 32 s), s the reference's own SIMD-vs-scalar
+    difference on that matrix, which reaches 1e-11 .. 1e-10 on the general mix's ill-conditioned tail)"""
+    import torch
+    import matfun_amd
+    from oracle.pyoracle import Checker, available
+    from tests.parity import spread_check
+    lib = matfun_amd.library()
+    ref = Checker("ref") if available("ref") else None
+    ref = None if ref is None or ref.name().endswith("(scalar)") else ref
+    n = 1 << 20
+    G = W.general(n, seed=0)
+    dG = torch.from_numpy(G).cuda()
+    rows = np.unique(np.concatenate([np.linspace(0, n - 1, 4096).astype(np.int64), np.arange(n - 64, n)]))
+    drows = torch.from_numpy(rows).cuda()
+    nb = lib.gmf_hip_sort_workspace(n)
+    work = torch.empty(nb, dtype=torch.uint8, device="cuda")
+    F = torch.empty((n, 9), dtype=torch.float64, device="cuda")
+    J = torch.empty((n, 81), dtype=torch.float64, device="cuda")
+    H = torch.empty((n, 729), dtype=torch.float64, device="cuda")
+    st = torch.cuda.current_stream().cuda_stream
+    oracle = Checker("oracle")
+    for kind, alpha in (("log", 0.0), ("pow", 0.3)):
+        print(f"full size sorted gmf {kind} order 2, n = {n}", flush=True)
+        k = matfun_amd.KINDS[kind]
+        assert lib.gmf_hip_eval_sorted(k, alpha, n, dG.data_ptr(), F.data_ptr(), J.data_ptr(), H.data_ptr(),
+                                       work.data_ptr(), nb, st) == 0
+        torch.cuda.synchronize()
+        got = [t.index_select(0, drows).cpu().numpy() for t in (F, J, H)]
+        # the same rows through gmf_hip_eval (the unsorted kernel at the same order): bitwise equal
+        from tests.gpu_util import call
+        rc, Fp, Jp, Hp = call("general", kind, alpha, G[rows], 2)
+        assert rc == 0
+        same(got, (Fp, Jp, Hp))
+        _, Fo, Jo, Ho = oracle.gmf(kind, alpha, G[rows], 2)
+        rr = ref.gmf(kind, alpha, G[rows], 2) if ref is not None else (0, None, None, None)
+        for name, x, y, z in zip("FJH", got, (Fo, Jo, Ho), rr[1:]):
+            spread_check(f"full size sorted {kind} {alpha} o2 {name}", x, y, z)

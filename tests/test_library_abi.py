@@ -116,3 +116,49 @@ def test_plan_rejects_invalid_groups_and_2_31_matrices_before_device_work():
     assert f(bad, 2, ctypes.byref(plan)) == -1 and not plan.value
     neg = (Group * 1)(Group(1, 2, 1, 10, float("nan")))                            # pow with a NaN exponent
     assert f(neg, 1, ctypes.byref(plan)) == -1 and not plan.value
+
+
+@pytest.mark.skipif(not os.path.exists(LIB), reason="HIP library not built (python -m matfun_amd.build)")
+def test_sorted_entry_points_reject_before_device_work():
+    """gmf_hip_eval_sorted*: gmf_hip_eval's checks, plus -1 for a missing or too small workspace (fewer than
+    min(n, 64) matrices), all before anything is launched; the workspace size grows with n"""
+    lib = ctypes.CDLL(LIB)
+    lib.gmf_hip_sort_workspace.argtypes, lib.gmf_hip_sort_workspace.restype = [ctypes.c_int64], ctypes.c_size_t
+    ws = lib.gmf_hip_sort_workspace
+    assert ws(-1) == 0 and ws(64) < ws(65) < ws(1 << 20) and ws(1 << 20) < 6 * (1 << 20)
+    dummy = ctypes.c_void_p(0x1000)
+    big = 1 << 40
+    args = [ctypes.c_int, ctypes.c_double, ctypes.c_int64] + [ctypes.c_void_p] * 5 + [ctypes.c_size_t, ctypes.c_void_p]
+    for f in (lib.gmf_hip_eval_sorted, lib.gmf_hip_eval_sorted_mixed):
+        f.argtypes, f.restype = args, ctypes.c_int
+        assert f(9, 0.0, 10, dummy, dummy, None, None, dummy, big, None) == -1           # unknown kind
+        assert f(2, 0.0, 10, dummy, dummy, None, None, dummy, big, None) == -1           # pow needs alpha > 0
+        assert f(0, 0.0, 10, dummy, dummy, None, dummy, dummy, big, None) == -1          # H without J
+        assert f(0, 0.0, -1, dummy, dummy, None, None, dummy, big, None) == -1           # n < 0
+        assert f(0, 0.0, 10, dummy, dummy, None, None, None, big, None) == -1            # no workspace
+        assert f(0, 0.0, 10, dummy, dummy, None, None, dummy, ws(64) - 1, None) == -1    # holds fewer than 10
+        assert f(0, 0.0, 0, None, None, None, None, None, 0, None) == 0                  # n == 0
+        assert f(0, 0.0, (2 ** 31 - 1) * 64 + 1, dummy, dummy, None, None, dummy, big, None) == -2
+    pargs = [ctypes.c_void_p, ctypes.c_int64] + [ctypes.c_void_p] * 5 + [ctypes.c_size_t, ctypes.c_void_p]
+    for f in (lib.gmf_hip_eval_pow_sorted, lib.gmf_hip_eval_pow_sorted_mixed):
+        f.argtypes, f.restype = pargs, ctypes.c_int
+        assert f(None, 10, dummy, dummy, None, None, dummy, big, None) == -1             # no exponents
+        assert f(dummy, 10, dummy, dummy, None, dummy, dummy, big, None) == -1           # H without J
+        assert f(dummy, 100, dummy, dummy, None, None, dummy, ws(64) - 1, None) == -1    # workspace below 64
+        assert f(None, 0, None, None, None, None, None, 0, None) == 0
+
+
+@pytest.mark.skipif(not os.path.exists(LIB), reason="HIP library not built (python -m matfun_amd.build)")
+def test_sorted_workspace_of_n_always_suffices():
+    """the workspace rule at its boundary: one byte less than gmf_hip_sort_workspace(min(n, 64)) is refused for every
+    n (only rejections are called here: dummy pointers must never reach a launch; the GPU tests run the accepted
+    sizes, tests/test_gpu_sorted.py)"""
+    lib = ctypes.CDLL(LIB)
+    lib.gmf_hip_sort_workspace.argtypes, lib.gmf_hip_sort_workspace.restype = [ctypes.c_int64], ctypes.c_size_t
+    f = lib.gmf_hip_eval_sorted
+    f.argtypes = [ctypes.c_int, ctypes.c_double, ctypes.c_int64] + [ctypes.c_void_p] * 5 + [ctypes.c_size_t,
+                                                                                            ctypes.c_void_p]
+    dummy = ctypes.c_void_p(0x1000)
+    for n in (1, 2, 63, 64, 65, 1000):
+        small = lib.gmf_hip_sort_workspace(min(n, 64)) - 1
+        assert f(0, 0.0, n, dummy, dummy, None, None, dummy, small, None) == -1
