@@ -18,6 +18,7 @@ with per-matrix exponents / gmf exp o1 / gmf log o2) evaluated by one kernel, 20
 replay, launch strategy tuned, FP64 and mixed (hetero_batch's docstring).
 
     python bench.py [--steps K] [--warmup W] [--n N] [--gmf-n N] [--gpus 1] [--no-cpu] [--only-headline]
+                    [--only-order0]
                     [--mixed-n N] [--mixed-gmf-n N] [--batch-elements E] [--batch-iters I]
 
 `value` = matrices/s of the headline step (4 n matrices per step).  `roofline`: the dominant kernel (the smf order-2
@@ -63,6 +64,21 @@ def profiled_traffic(n):
     return None
 
 
+def profiled_valu():
+    """VALU figures of the order-0 kernels from the committed rocprofv3 summary (profiles/order0_valu.md, lines
+    `valu <kernel>: <key>=<value> ...`), or None"""
+    path = os.path.join(ROOT, "profiles", "order0_valu.md")
+    out = {}
+    try:
+        for line in open(path):
+            if line.startswith("valu "):
+                name, rest = line[5:].split(":", 1)
+                out[name.strip()] = {k: float(v) for k, v in (kv.split("=") for kv in rest.split())}
+    except OSError:
+        return None
+    return out or None
+
+
 def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
@@ -72,7 +88,7 @@ def cpu_baseline(A, seconds=10.0, chunk=1 << 17):
     from oracle.pyoracle import Checker, available
     kind = "reference" if available("ref") else "port"
     chk = Checker("ref" if kind == "reference" else "oracle")
-    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    threads = int(os.environ.get("OMP_NUM_THREADS", len(os.sched_getaffinity(0))))
     chk.set_threads(threads)
     done, lo = 0, 0
     chk.smf_matrix("log", 0.0, A[:256], 2)                         # warm-up (libmvec binding, thread pool)
@@ -85,9 +101,46 @@ def cpu_baseline(A, seconds=10.0, chunk=1 << 17):
             done += len(X)
         lo += chunk
     dt = time.perf_counter() - t0
-    return {"value": done / dt, "unit": "matrices/s", "cores": threads, "kind": kind,
+    return {"value": done / dt, "unit": "matrices/s", "cores": threads, "kind": kind, "cpu": cpu_model(),
+            "host_threads_available": len(os.sched_getaffinity(0)),
             "sample": f"{done} matrices ({done // 4} per function, first rows of the headline inputs), {chk.name()}, "
                       f"order 2 matrix layout, {dt:.1f} s wall"}
+
+
+def cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def c0_baseline(runs=7, n=65536):
+    """C0: smf log, Voigt layout, order 1 (F n x 6, J n x 6 x 6), n = 65536 SPD matrices (Q Haar-random from seed 0,
+    eigenvalues log-uniform in [1e-2, 1e2], no repeated ones) on the reference C path (c/src/matfun.cc:87-89: the
+    reference library built from its sources, SIMD + OpenMP on all threads given) -- median of `runs` timed calls
+    after a warm-up"""
+    from oracle.pyoracle import Checker, available
+    from matfun_amd import workloads as W
+    kind = "reference" if available("ref") else "port"
+    chk = Checker("ref" if kind == "reference" else "oracle")
+    threads = int(os.environ.get("OMP_NUM_THREADS", len(os.sched_getaffinity(0))))
+    chk.set_threads(threads)
+    v = W.voigt(W.spd(n, seed=0, special=False))
+    rc, *_ = chk.smf("log", 0.0, v, 1)                             # warm-up
+    assert rc == 0
+    ts = []
+    for _ in range(runs):
+        t0 = time.perf_counter()
+        rc, *_ = chk.smf("log", 0.0, v, 1)
+        ts.append(time.perf_counter() - t0)
+        assert rc == 0
+    med = float(np.median(ts))
+    return {"value": n / med, "unit": "matrices/s", "cores": threads, "kind": kind, "cpu": cpu_model(),
+            "ms_median": med * 1e3, "runs": runs, "n": n,
+            "sample": f"C0: smf log, Voigt, order 1, n={n}, {chk.name()}, median of {runs} after a warm-up"}, v
 
 
 def timed(torch, stream, calls, steps, warmup):
@@ -112,7 +165,8 @@ def timed(torch, stream, calls, steps, warmup):
 
 
 def mixed_vs_fp64(torch, lib, W, dev, stream, args, sample=1 << 16):
-    """smf log + smf pow (per-matrix exponent ~ U[-1, 1]) order 2 and gmf log order 1, FP64 form vs mixed form"""
+    """smf log + smf pow (per-matrix exponent ~ U[-1, 1]) order 2 and gmf log order 1 (the length-sorted entry
+    point, faster at order 1), FP64 form vs mixed form"""
     import matfun_amd
     n, ng = args.mixed_n, args.mixed_gmf_n
     log(f"bench: mixed vs fp64: generating {n} symmetric and {ng} general inputs (seed 1)")
@@ -120,8 +174,9 @@ def mixed_vs_fp64(torch, lib, W, dev, stream, args, sample=1 << 16):
     dal = torch.from_numpy(np.random.default_rng(1).uniform(-1.0, 1.0, n)).to(dev)
     dG = torch.from_numpy(W.general(ng, seed=1)).to(dev)
     sh = stream.cuda_stream
+    work = torch.empty(lib.gmf_hip_sort_workspace(ng), dtype=torch.uint8, device=dev)
     res = {"workload": f"smf log + smf pow(alpha per matrix ~ U[-1,1]), matrix layout, order 2, n={n} each; gmf log "
-                       f"order 1, n={ng}", "n_smf": n, "n_gmf": ng}
+                       f"order 1, n={ng} (gmf_hip_eval_sorted*)", "n_smf": n, "n_gmf": ng}
     t = {}
     for form, dt, eb in (("fp64", torch.float64, 8), ("mixed", torch.float32, 4)):
         mixed = form == "mixed"
@@ -131,10 +186,11 @@ def mixed_vs_fp64(torch, lib, W, dev, stream, args, sample=1 << 16):
         p = (F.data_ptr(), J.data_ptr(), H.data_ptr())
         smf = lib.smf_hip_eval_matrix_mixed if mixed else lib.smf_hip_eval_matrix
         smfp = lib.smf_hip_eval_matrix_pow_mixed if mixed else lib.smf_hip_eval_matrix_pow
-        gmf = lib.gmf_hip_eval_mixed if mixed else lib.gmf_hip_eval
+        gmf = lib.gmf_hip_eval_sorted_mixed if mixed else lib.gmf_hip_eval_sorted
         calls = [lambda: smf(matfun_amd.KINDS["log"], 0.0, n, dA.data_ptr(), *p, sh),
                  lambda: smfp(dal.data_ptr(), n, dA.data_ptr(), *p, sh),
-                 lambda: gmf(matfun_amd.KINDS["log"], 0.0, ng, dG.data_ptr(), p[0], p[1], None, sh)]
+                 lambda: gmf(matfun_amd.KINDS["log"], 0.0, ng, dG.data_ptr(), p[0], p[1], None, work.data_ptr(),
+                             work.numel(), sh)]
         ks, wall = timed(torch, stream, calls, args.steps, args.warmup)
         b2, b1 = bytes_per_matrix(2, eb), bytes_per_matrix(1, eb)
         res[form] = {"ms_per_step": wall * 1e3, "avg_kernel_ms": {"smf_log": ks[0], "smf_pow_alpha": ks[1],
@@ -176,7 +232,7 @@ def mixed_vs_fp64(torch, lib, W, dev, stream, args, sample=1 << 16):
         err[name] = e
     res["mixed_err_vs_fp64"] = err
     log(f"bench: mixed vs fp64: speed-up {res['speedup_mixed_vs_fp64']:.3f}")
-    del dA, dal, dG
+    del dA, dal, dG, work
     torch.cuda.empty_cache()
     return res
 
@@ -256,6 +312,7 @@ def main():
     ap.add_argument("--gmf-n", type=int, default=1 << 20)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--only-headline", action="store_true")
+    ap.add_argument("--only-order0", action="store_true", help="only the order-0 legs (for the VALU counter runs)")
     ap.add_argument("--only-hencky", action="store_true", help="the headline and the Hencky legs only")
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--mixed-n", type=int, default=1 << 24)
@@ -296,15 +353,18 @@ def main():
     stream = torch.cuda.current_stream()
     sh = stream.cuda_stream
 
-    def leg(name, fn, X, m, out_bytes, steps, warmup, funcs=FUNCS, bpm=None):
+    def leg(name, fn, X, m, out_bytes, steps, warmup, funcs=FUNCS, bpm=None, order=2, work=None):
+        """one call per function in `funcs` per step; order 0 / 1 / 2 (F; F, J; F, J, H); work: the workspace
+        (tensor) of the length-sorted entry points"""
         ev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in funcs]
-        ptrs = (Fb.data_ptr(), Jb.data_ptr(), Hb.data_ptr())
+        ptrs = (Fb.data_ptr(), Jb.data_ptr() if order >= 1 else None, Hb.data_ptr() if order >= 2 else None)
+        ws = (work.data_ptr(), work.numel()) if work is not None else ()
 
         def step(record):
             for i, (k, al) in enumerate(funcs):
                 if record:
                     ev[i][0].record(stream)
-                rc = fn(matfun_amd.KINDS[k], al, m, X.data_ptr(), *ptrs, sh)
+                rc = fn(matfun_amd.KINDS[k], al, m, X.data_ptr(), *ptrs, *ws, sh)
                 if record:
                     ev[i][1].record(stream)
                 assert rc == 0, (name, rc)
@@ -320,13 +380,13 @@ def main():
             kern += [a.elapsed_time(b) for a, b in ev]
         wall = (time.perf_counter() - t0) / steps
         kms = float(np.mean(kern))
-        bpm = bpm or bytes_per_matrix(2, out_bytes)
+        bpm = bpm or bytes_per_matrix(order, out_bytes)
         gbs = m * bpm / (kms * 1e-3) / 1e9
         nf = len(funcs)
         per = {fname(k, al): m / (float(np.mean(kern[i::nf])) * 1e-3) for i, (k, al) in enumerate(funcs)}
         log(f"bench: {name}: {wall * 1e3:.2f} ms/step, kernel avg {kms:.3f} ms, {gbs:.0f} GB/s "
             f"({gbs / HBM_PEAK_GBS:.1%} of HBM peak), {nf * m / wall / 1e6:.1f} M matrices/s")
-        return {"functions": [fname(k, al) for k, al in funcs], "matrices_per_s": nf * m / wall,
+        return {"functions": [fname(k, al) for k, al in funcs], "order": order, "matrices_per_s": nf * m / wall,
                 "ms_per_step": wall * 1e3, "n_per_function": m, "bytes_per_matrix": bpm, "avg_kernel_ms": kms,
                 "achieved_GBs": gbs, "frac": gbs / HBM_PEAK_GBS, "per_function_matrices_per_s": per}
 
@@ -422,6 +482,23 @@ def main():
         return res
 
     funcs_mx = FUNCS[1:]          # (exp of eigenvalues up to 1e2 is beyond the FP32 range of the mixed form)
+
+    def order0_legs():
+        """order 0 (F only: 144 B per matrix FP64, 108 B mixed): bound by the per-matrix arithmetic (FP64 square roots,
+        trigonometric root, transcendentals), not by HBM; the VALU counters of these kernels are read from
+        profiles/order0_valu.md"""
+        o0 = {"smf_fp64": leg("smf fp64 order 0", lib.smf_hip_eval_matrix, dA, n, 8, args.steps, args.warmup, order=0),
+              "smf_mixed": leg("smf mixed order 0", lib.smf_hip_eval_matrix_mixed, dA, n, 4, args.steps, args.warmup,
+                               funcs_mx, order=0),
+              "gmf_fp64": leg("gmf fp64 order 0", lib.gmf_hip_eval, dG, ng, 8, args.steps, args.warmup, order=0),
+              "gmf_mixed": leg("gmf mixed order 0", lib.gmf_hip_eval_mixed, dG, ng, 4, args.steps, args.warmup,
+                               order=0)}
+        o0["valu"] = profiled_valu()
+        return o0
+
+    if args.only_order0:
+        print(json.dumps({"order0": order0_legs()}), flush=True)
+        return
     head = leg("smf fp64", lib.smf_hip_eval_matrix, dA, n, 8, args.steps, args.warmup)
     legs = {"smf_fp64": head}
     if not (args.only_headline or args.only_hencky):
@@ -436,17 +513,33 @@ def main():
         del dV
         legs["gmf_fp64"] = leg("gmf fp64", lib.gmf_hip_eval, dG, ng, 8, args.steps, args.warmup)
         legs["gmf_mixed"] = leg("gmf mixed", lib.gmf_hip_eval_mixed, dG, ng, 4, args.steps, args.warmup)
+        # the same through the length-sorted entry points (workspace allocated once, outside the timed region)
+        work = torch.empty(lib.gmf_hip_sort_workspace(ng), dtype=torch.uint8, device=dev)
+        legs["gmf_fp64_sorted"] = leg("gmf fp64 sorted", lib.gmf_hip_eval_sorted, dG, ng, 8, args.steps, args.warmup,
+                                      work=work)
+        legs["gmf_mixed_sorted"] = leg("gmf mixed sorted", lib.gmf_hip_eval_sorted_mixed, dG, ng, 4, args.steps,
+                                       args.warmup, work=work)
+        legs["order0"] = order0_legs()
+        del work
         legs["smf_mixed"]["max_err_vs_fp64"] = mixed_error(lib.smf_hip_eval_matrix, lib.smf_hip_eval_matrix_mixed,
                                                            dA, funcs_mx)
         legs["gmf_mixed"]["max_err_vs_fp64"] = mixed_error(lib.gmf_hip_eval, lib.gmf_hip_eval_mixed, dG, FUNCS)
-        # FP64 vs mixed on the same functions: summed time per step of the mixed legs' functions
+        # FP64 vs mixed on the same functions (smf log, sqrt, pow(0.3) -- exp of the headline eigenvalues is beyond the
+        # FP32 range -- and gmf exp, log, sqrt, pow(0.3)): summed kernel time per function, each form through its
+        # faster general-matrix entry point (plain or length-sorted) per function
         t64_smf = sum(n / head["per_function_matrices_per_s"][fname(k, al)] for k, al in funcs_mx)
-        tmx_smf = len(funcs_mx) * n / legs["smf_mixed"]["matrices_per_s"]
-        t64 = t64_smf + 4 * ng / legs["gmf_fp64"]["matrices_per_s"]
-        tmx = tmx_smf + 4 * ng / legs["gmf_mixed"]["matrices_per_s"]
+        tmx_smf = sum(n / legs["smf_mixed"]["per_function_matrices_per_s"][fname(k, al)] for k, al in funcs_mx)
+
+        def gmf_time(form):
+            a, b = legs[f"gmf_{form}"], legs[f"gmf_{form}_sorted"]
+            return sum(ng / max(a["per_function_matrices_per_s"][f], b["per_function_matrices_per_s"][f])
+                       for f in a["per_function_matrices_per_s"])
+        t64, tmx = t64_smf + gmf_time("fp64"), tmx_smf + gmf_time("mixed")
         legs["smf_mixed"]["speedup_vs_fp64"] = t64_smf / tmx_smf
-        legs["gmf_mixed"]["speedup_vs_fp64"] = legs["gmf_mixed"]["matrices_per_s"] / legs["gmf_fp64"]["matrices_per_s"]
+        legs["gmf_mixed"]["speedup_vs_fp64"] = gmf_time("fp64") / gmf_time("mixed")
         legs["mixed_vs_fp64_smf_plus_gmf"] = t64 / tmx
+        legs["mixed_vs_fp64_smf_plus_gmf_functions"] = ("smf " + ", ".join(fname(k, al) for k, al in funcs_mx) +
+                                                        "; gmf exp, log, sqrt, pow0.3 (faster of plain and sorted)")
 
     if not args.only_headline:
         legs["hencky_fp64"] = hencky_leg("hencky fp64", lib.mf_hip_hencky, torch.float64, args.steps, args.warmup)
@@ -490,6 +583,17 @@ def main():
     if not args.no_cpu:
         log("bench: CPU baseline")
         out["cpu_baseline"] = cpu_baseline(A, seconds=args.cpu_seconds)
+        c0, v0 = c0_baseline()
+        # the HIP path on the same C0 inputs (smf_hip_eval, Voigt, order 1; inputs and outputs resident on the GPU)
+        dv = torch.from_numpy(v0).to(dev)
+        F0 = torch.empty(len(v0) * 6, dtype=torch.float64, device=dev)
+        J0 = torch.empty(len(v0) * 36, dtype=torch.float64, device=dev)
+        ks, _ = timed(torch, stream, [lambda: lib.smf_hip_eval(1, 0.0, len(v0), dv.data_ptr(), F0.data_ptr(),
+                                                               J0.data_ptr(), None, sh)], 20, 3)
+        c0["hip_same_inputs"] = {"matrices_per_s": len(v0) / (ks[0] * 1e-3), "avg_call_ms": ks[0]}
+        out["cpu_baseline"]["c0"] = c0
+        log(f"bench: C0 (smf log Voigt order 1, n = {len(v0)}): CPU {c0['value'] / 1e6:.2f} M matrices/s on "
+            f"{c0['cores']} threads ({c0['cpu']}); HIP {len(v0) / (ks[0] * 1e-3) / 1e6:.1f} M matrices/s")
     print(json.dumps(out), flush=True)
 
 
