@@ -89,6 +89,14 @@ int gmf_hip_eval_pow_sorted(const double *alpha, int64_t n, const double *A, dou
                             void *work, size_t work_bytes, void *stream);
 int gmf_hip_eval_pow_sorted_mixed(const double *alpha, int64_t n, const double *A, float *F, float *J, float *H,
                                   void *work, size_t work_bytes, void *stream);
+/* gmf_hip_eval_sorted without the workspace argument, for callers that cannot allocate device memory (code under
+ * hipGraph capture, tight serving loops, harnesses that forbid hipMalloc): no counterpart in the reference.  The
+ * workspace is a static device array of the library (21.2 MB on each device the library runs on, held until the
+ * process ends); nothing is allocated per call.  At most 2^22 matrices per call (-1 beyond: split the batch); the
+ * same results (bitwise), arguments and return codes as gmf_hip_eval_sorted otherwise, FP64.  Calls that overlap
+ * on one device share the array and race: issue them on one stream (or serialise them). */
+int gmf_hip_eval_sorted_noalloc(int kind, double alpha, int64_t n, const double *A, double *F, double *J, double *H,
+                                void *stream);
 
 /* ---- packed symmetric outputs (no counterpart in the reference API; the H packing is the reference kernel's own
  * HPACK layout, c/include/matfun/smf_kernel.hpp:196-199): Voigt input v (n,6) as smf_hip_eval, outputs of the

@@ -59,6 +59,7 @@ def library():
         for fn in (lib.gmf_hip_eval_pow_sorted, lib.gmf_hip_eval_pow_sorted_mixed):
             fn.argtypes, fn.restype = spargs, ctypes.c_int
         lib.gmf_hip_sort_workspace.argtypes, lib.gmf_hip_sort_workspace.restype = [ctypes.c_int64], ctypes.c_size_t
+        lib.gmf_hip_eval_sorted_noalloc.argtypes, lib.gmf_hip_eval_sorted_noalloc.restype = args, ctypes.c_int
         for fn in (lib.mf_hip_hencky, lib.mf_hip_hencky_mixed):
             fn.argtypes, fn.restype = [ctypes.c_int64] + [ctypes.c_void_p] * 7, ctypes.c_int
         for fn in (lib.smf_hip_hencky_matrix, lib.smf_hip_hencky, lib.gmf_hip_hencky):

@@ -16,6 +16,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <mutex>
+
 #include "gmf_sorted.hpp"
 #include "gmf_tile.hpp"
 
@@ -237,3 +239,53 @@ template int gmf_sorted_launch<float>(hipStream_t, int, int, int64_t, const doub
                                       const double*, const GmfCoef&, void*, int64_t);
 
 }  // namespace mf
+
+// ---- gmf_hip_eval_sorted_noalloc: the length-sorted evaluation with its workspace in a static device array, for
+// callers that cannot allocate (graph capture, serving loops): at most STATIC_MAX_N matrices per call.  Each device
+// holds its own copy of `scratch` (21.2 MB) from the first call on it until the process ends.  The kernels write every
+// byte of the workspace they read (keys, histograms, offsets, permutation), so nothing is cleared between calls.
+// Calls that overlap on one device (concurrent streams, threads) share the array and race; calls on one stream
+// follow each other.
+namespace mf {
+namespace {
+constexpr int64_t STATIC_MAX_N = (int64_t)1 << 22;
+constexpr size_t STATIC_BYTES = (size_t)STATIC_MAX_N + 4 * (size_t)STATIC_MAX_N +
+                                2 * (((size_t)4 * KSORT_MAXCH * KBINS + 255) / 256 * 256);
+static __device__ __align__(256) unsigned char scratch[STATIC_BYTES];
+
+constexpr int MAX_DEVICES = 64;
+std::mutex g_scratch_mutex;
+void* g_scratch_ptr[MAX_DEVICES] = {};
+
+// the device address of the current device's copy of `scratch` (looked up once per device), or nullptr
+void* scratch_of_current_device() {
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= MAX_DEVICES) return nullptr;
+  std::lock_guard<std::mutex> lock(g_scratch_mutex);
+  if (!g_scratch_ptr[dev]) {
+    void* ptr = nullptr;
+    if (hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)) != hipSuccess || !ptr) return nullptr;
+    g_scratch_ptr[dev] = ptr;
+  }
+  return g_scratch_ptr[dev];
+}
+}  // namespace
+
+}  // namespace mf
+
+extern "C" int gmf_hip_eval_sorted_noalloc(int kind, double alpha, int64_t n, const double* A, double* F, double* J,
+                                           double* H, void* stream) {
+  using namespace mf;
+  if (kind < EXP || kind > SQRT || n < 0 || (H && !J) || (kind == POW && !(alpha > 0))) return -1;
+  if (n > STATIC_MAX_N) return -1;                        // (beyond the static workspace: split the batch)
+  if (kind == SQRT) kind = POW, alpha = 0.5;
+  if (kind != POW) alpha = 0.0;
+  if (n == 0) return 0;
+  void* work = scratch_of_current_device();
+  if (!work) return -2;
+  GmfCoef bt;
+  gmf_coefs(kind, alpha, bt);
+  const int order = H ? 2 : (J ? 1 : 0);
+  return gmf_sorted_launch<double>((hipStream_t)stream, kind, order, n, A, F, J, H, alpha, nullptr, bt, work,
+                                   STATIC_MAX_N);
+}
