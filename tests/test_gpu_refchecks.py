@@ -56,3 +56,22 @@ def test_rotations_near_180(ev, deg):
 
 def test_pow_large_alpha_against_mpmath(ev, mp_refs):
     R.pow_large_alpha(ev, mp_refs)
+
+
+def test_against_high_precision_fixture(ev):
+    """the GPU against the independent 60 / 120-digit reference (tests/golden/mpref.npz; tests/test_oracle_mpref.py
+    pins the oracle to it at 1e-13): every class of both benchmark distributions, F / J / H, bound 1e-13 per
+    matrix; the per-class maxima are printed"""
+    import os
+    import numpy as np
+    from tests.test_oracle_mpref import GOLD, errors
+    d = np.load(GOLD)
+    worst = {}
+    for fam, f in (("smf", lambda k, al, x: ev.smf(k, al, x, 2)), ("gmf", lambda k, al, x: ev.gmf(k, al, x, 2))):
+        for label, k, e in errors(d, fam, f):
+            key = f"{fam} {k} {label}"
+            worst[key] = [max(a, b) for a, b in zip(worst.get(key, [0, 0, 0]), e)]
+    for key, e in sorted(worst.items()):
+        print(f"MPREF {key}: F {e[0]:.1e} J {e[1]:.1e} H {e[2]:.1e}", flush=True)
+    bad = {k: e for k, e in worst.items() if max(e) > 1e-13}
+    assert not bad, bad
