@@ -69,16 +69,22 @@ int smf_hip_eval_matrix_pow_mixed(const double *alpha, int64_t n, const double *
 int gmf_hip_eval_pow_mixed(const double *alpha, int64_t n, const double *A, float *F, float *J, float *H,
                            void *stream);
 
-/* ---- general matrices ordered by series length (no counterpart in the reference API): the results of gmf_hip_eval /
- * gmf_hip_eval_pow (and their _mixed forms) bitwise, faster on batches whose matrices need group series of very
- * different lengths.  A wavefront of 64 matrices runs its series loop as long as its longest series (on the general
- * workload mix the longest of 64 averages 3x the mean), so the matrices are first sorted by series length on the
- * device (a length pass, a counting sort into a permutation) and then evaluated in that order; every matrix still
- * gets its own results written to its own rows.  `work` is caller-owned device memory of work_bytes bytes,
- * gmf_hip_sort_workspace(n) for the whole batch in one pass (about 5 bytes per matrix plus 200 KB); a smaller
- * workspace runs the batch in consecutive parts of what it holds.  Same arguments, layouts, return codes and
- * asynchrony as gmf_hip_eval, plus -1 when n > 0 and the workspace is NULL or holds fewer than min(n, 64) matrices
- * (gmf_hip_sort_workspace(n) bytes always suffice).  Calls
+/* ---- general matrices with a workspace, evaluated in the order of their series lengths (no counterpart in the
+ * reference API): gmf_hip_eval / gmf_hip_eval_pow (and their _mixed forms) for batches whose matrices need group series
+ * of very different lengths.  A wavefront of 64 matrices runs its series loop as long as its longest series (on the
+ * general workload mix the longest of 64 averages 3x the mean), so the library may sort the matrices by series length
+ * on the device first.  How, per function (kind 0..3; 4 = per-matrix exponents), order and precision, is a
+ * process-wide strategy (gmf_hip_set_sort_strategy; defaults measured on MI355X, matfun_amd.tune times them):
+ *   0  no sorting: the kernel of gmf_hip_eval (bitwise its results)
+ *   1  the matrices in length order through the same kernel, each writing its own rows (bitwise its results)
+ *   2  staged: per matrix its nodes and series inputs (input order), the series in length order, the rest in input
+ *      order with contiguous stores -- gmf_hip_eval's results up to rounding (per matrix normwise <= 1e-13; measured
+ *      at most 5e-14 on the general mix, with the same error distribution against the reference)
+ * `work` is caller-owned device memory of work_bytes bytes; gmf_hip_sort_workspace(n) bytes (about 540 bytes per
+ * matrix plus 200 KB) take any batch of n in one pass; a smaller workspace runs the batch in consecutive parts (a
+ * staged strategy uses the permuted one when the workspace holds fewer than 2^15 matrices of it).  Same arguments,
+ * layouts, return codes and asynchrony as gmf_hip_eval, plus -1 when n > 0 and the workspace is NULL or holds fewer
+ * than min(n, 64) matrices of the permuted strategy (gmf_hip_sort_workspace(min(n, 64)) bytes always suffice).  Calls
  * that share a workspace must not overlap (stream order suffices). */
 size_t gmf_hip_sort_workspace(int64_t n);
 int gmf_hip_eval_sorted(int kind, double alpha, int64_t n, const double *A, double *F, double *J, double *H, void *work,
@@ -93,10 +99,15 @@ int gmf_hip_eval_pow_sorted_mixed(const double *alpha, int64_t n, const double *
  * hipGraph capture, tight serving loops, harnesses that forbid hipMalloc): no counterpart in the reference.  The
  * workspace is a static device array of the library (21.2 MB on each device the library runs on, held until the
  * process ends); nothing is allocated per call.  At most 2^22 matrices per call (-1 beyond: split the batch); the
- * same results (bitwise), arguments and return codes as gmf_hip_eval_sorted otherwise, FP64.  Calls that overlap
- * on one device share the array and race: issue them on one stream (or serialise them). */
+ * same strategies, results (bitwise: a matrix's results do not depend on the parts a batch runs in), arguments and
+ * return codes as gmf_hip_eval_sorted otherwise, FP64.  Calls that overlap on one device share the array and race:
+ * issue them on one stream (or serialise them). */
 int gmf_hip_eval_sorted_noalloc(int kind, double alpha, int64_t n, const double *A, double *F, double *J, double *H,
                                 void *stream);
+/* the strategy of gmf_hip_eval_sorted* for a function (kind 0..3; 4 = per-matrix exponents), order (0..2) and
+ * precision (mixed 0 FP64, 1 FP32 outputs): 0, 1, 2 as above.  Returns 0 / -1 (set), the strategy or -1 (get). */
+int gmf_hip_set_sort_strategy(int kind, int order, int mixed, int strategy);
+int gmf_hip_get_sort_strategy(int kind, int order, int mixed);
 
 /* ---- packed symmetric outputs (no counterpart in the reference API; the H packing is the reference kernel's own
  * HPACK layout, c/include/matfun/smf_kernel.hpp:196-199): Voigt input v (n,6) as smf_hip_eval, outputs of the

@@ -60,6 +60,8 @@ def library():
             fn.argtypes, fn.restype = spargs, ctypes.c_int
         lib.gmf_hip_sort_workspace.argtypes, lib.gmf_hip_sort_workspace.restype = [ctypes.c_int64], ctypes.c_size_t
         lib.gmf_hip_eval_sorted_noalloc.argtypes, lib.gmf_hip_eval_sorted_noalloc.restype = args, ctypes.c_int
+        lib.gmf_hip_set_sort_strategy.argtypes = [ctypes.c_int] * 4
+        lib.gmf_hip_get_sort_strategy.argtypes = [ctypes.c_int] * 3
         for fn in (lib.mf_hip_hencky, lib.mf_hip_hencky_mixed):
             fn.argtypes, fn.restype = [ctypes.c_int64] + [ctypes.c_void_p] * 7, ctypes.c_int
         for fn in (lib.smf_hip_hencky_matrix, lib.smf_hip_hencky, lib.gmf_hip_hencky):
@@ -218,8 +220,10 @@ def gmf(kind, A, alpha=None, order=0, precision="fp64", sort=False):
     """f(A) and its derivatives for general real 3x3 matrices A (..., 3, 3), principal branch; NaN results for a
     matrix with a real eigenvalue <= 0 (log, sqrt, pow).  order 0: F, 1: (F, J), 2: (F, J, H).
     precision "mixed": FP32 results (FP64 per-matrix spectral part, FP32 assembly of J and H).
-    sort=True: evaluated in the order of the matrices' series lengths (gmf_hip_eval_sorted*, a workspace of about
-    5 bytes per matrix from torch's allocator): the same results bitwise, faster on mixed batches."""
+    sort=True: through the workspace entry points (gmf_hip_eval_sorted*, a workspace of about 540 bytes per matrix
+    from torch's allocator) that may evaluate the matrices in the order of their series lengths (the process-wide
+    strategy per function, order and precision, gmf_hip_set_sort_strategy): gmf_hip_eval's results bitwise or, for a
+    staged strategy, up to rounding (<= 1e-13 per matrix)."""
     _prepare(kind, alpha, order)
     mixed, dt = _precision(precision)
     lib = library()

@@ -277,10 +277,18 @@ struct GmfSeriesLocal {
   MF_DEV void operator()(const GmfSeriesIn& s, cpx* acc) const { gmf_series<KIND, ORDER, PA>(s, bt, acc); }
 };
 
-// no series: gmf_lane stops once the series length is known and returns it in GmfOut::K (the first pass of the
-// length-sorted evaluation, gmf_sorted.hip)
+// no series: gmf_lane stops once the series inputs are known, returns the length in GmfOut::K and hands the inputs
+// and the nodes to post() (the first pass of the length-sorted evaluation, gmf_sorted.hip; NULL: not stored)
+struct GmfNodes;
 struct GmfSeriesLength {
   static constexpr bool K_ONLY = true;
+  GmfSeriesIn* out = nullptr;
+  GmfNodes* nodes = nullptr;
+  template <class N>
+  MF_DEV void post(const GmfSeriesIn& s, const N& st) const {
+    if (out) *out = s;
+    if (nodes) *nodes = st;
+  }
   MF_DEV void operator()(const GmfSeriesIn&, cpx*) const {}
 };
 
@@ -294,13 +302,21 @@ struct GmfOut {
   int K;              // series length (set only by a GmfSeriesLength pass)
 };
 
+// what the rest of a matrix's evaluation needs from its nodes, once the group series is set up (gmf_nodes ->
+// gmf_finish): the node offsets x from m and the nodes mu, the scaled mean m, the scale sig, the exp shift, the group
+// centre cxg and the series scale L; flags: bits 0..2 the refined nodes (ab), 3..5 the group (inG), 6 outside the
+// domain (bad), 7 a group exists (hasg)
+struct GmfNodes {
+  cpx x[3], mu[3];
+  double m, sig, xshift, cxg, L;
+  unsigned flags, pad;
+};
+
 // PA (powers with a per-matrix exponent al): bt holds 1 / j instead of the coefficients b_j (gmf_recips)
-// rinv: gmf_recip_table(); series(in, acc): the group series (GmfSeriesLocal, or one shared by the lanes of a
-// workgroup), called by every lane (K = 0 without a group)
-template <int KIND, int ORDER, bool PA, class SER>
-MF_DEV void gmf_lane(const double* a, double al, const double* __restrict__ bt, GmfOut& o,
-                     const double* __restrict__ rinv, const SER& series) {
-  constexpr int ND = ORDER == 0 ? 3 : (ORDER == 1 ? 9 : 19);
+// rinv: gmf_recip_table().  The nodes and the group series' inputs of one matrix (gmf_kernel.hpp:251-476)
+template <int KIND, int ORDER, bool PA>
+MF_DEV void gmf_nodes(const double* a, double al, const double* __restrict__ bt, const double* __restrict__ rinv,
+                      GmfNodes& st, GmfSeriesIn& sin) {
   const double RHO = 0.7, R_EXP = 4.0, RHO_POLE = 0.6;
   double amax = fabs(a[0]);
 #pragma unroll
@@ -526,16 +542,8 @@ MF_DEV void gmf_lane(const double* a, double al, const double* __restrict__ bt, 
   double rr = dmax(cabs_(u[0]), dmax(cabs_(u[1]), cabs_(u[2])));
   const double rq = ((pk[0] || pk[1] || pk[2]) && (rr * fabs(iap) <= 1.0)) ? rr * fabs(iap) : 0.0;
   rr = (hasg && rr <= R_EXP + 0.5) ? rr : 0.0;
-  cpx D[ND];
-#pragma unroll
-  for (int c = 0; c < ND; c++) D[c] = mk(0.0, 0.0);
   {
-    GmfSeriesIn sin;
     sin.K = hasg ? series_terms<PA>(bt, al, 3 * (ORDER + 1) - 1, rr, rq, rinv) : 0;
-    if constexpr (SER::K_ONLY) {
-      o.K = sin.K;
-      return;
-    }
     sin.b0 = (hasg && KIND == LOG) ? node_value<LOG>(al, sig, m, mk(cxg, 0.0), mk(cmu, 0.0), 0).re : 1.0;
 #pragma unroll
     for (int i = 0; i < 3; i++) sin.u[i] = u[i];
@@ -543,8 +551,44 @@ MF_DEV void gmf_lane(const double* a, double al, const double* __restrict__ bt, 
     sin.al = al;
     sin.flags = (inG[0] ? 1 : 0) | (inG[1] ? 2 : 0) | (inG[2] ? 4 : 0) | (pk[0] ? 8 : 0) | (pk[1] ? 16 : 0) |
                 (pk[2] ? 32 : 0);
-    cpx acc[ND];
-    series(sin, acc);
+  }
+#pragma unroll
+  for (int i = 0; i < 3; i++) { st.x[i] = x[i]; st.mu[i] = mu[i]; }
+  st.m = m;
+  st.sig = sig;
+  st.xshift = xshift;
+  st.cxg = cxg;
+  st.L = L;
+  st.flags = (ab[0] ? 1u : 0u) | (ab[1] ? 2u : 0u) | (ab[2] ? 4u : 0u) | (inG[0] ? 8u : 0u) | (inG[1] ? 16u : 0u) |
+             (inG[2] ? 32u : 0u) | (bad ? 64u : 0u) | (hasg ? 128u : 0u);
+  st.pad = 0;
+}
+
+// the rest of one matrix's evaluation from its nodes and group series sums acc (ND of them; gmf_kernel.hpp:478-671):
+// the group's divided differences, the single-node residues, the real basis and the contracted tensors
+template <int KIND, int ORDER, bool PA>
+MF_DEV void gmf_finish(const double* a, double al, const GmfNodes& st, const cpx* acc, GmfOut& o) {
+  constexpr int ND = ORDER == 0 ? 3 : (ORDER == 1 ? 9 : 19);
+  const double sig = st.sig, isig = 1.0 / sig, m = st.m, xshift = st.xshift, cxg = st.cxg, L = st.L, iL = 1.0 / L;
+  double As[9], S[9];
+#pragma unroll
+  for (int k = 0; k < 9; k++) As[k] = a[k] * isig;
+#pragma unroll
+  for (int k = 0; k < 9; k++) S[k] = k % 4 == 0 ? As[k] - m : As[k];
+  cpx x[3], mu[3];
+  bool ab[3], inG[3];
+#pragma unroll
+  for (int i = 0; i < 3; i++) {
+    x[i] = st.x[i];
+    mu[i] = st.mu[i];
+    ab[i] = (st.flags >> i) & 1u;
+    inG[i] = (st.flags >> (3 + i)) & 1u;
+  }
+  const bool bad = (st.flags & 64u) != 0, hasg = (st.flags & 128u) != 0;
+  cpx D[ND];
+#pragma unroll
+  for (int c = 0; c < ND; c++) D[c] = mk(0.0, 0.0);
+  {
     if (hasg) {
 #pragma unroll
       for (int c = 0; c < ND; c++) {
@@ -690,6 +734,27 @@ MF_DEV void gmf_lane(const double* a, double al, const double* __restrict__ bt, 
 #pragma unroll
     for (int k = 0; k < 27; k++) o.D3[k] = Dr[k];
   }
+}
+
+// one matrix: its nodes, the group series (series(in, acc): GmfSeriesLocal, one shared by the lanes of a workgroup,
+// or sums computed elsewhere; called by every lane, K = 0 without a group), the rest.  With a K_ONLY functor (the
+// first pass of the length-sorted evaluation) it stops after the nodes: the series length in o.K, the series inputs
+// and the nodes handed to series.post
+template <int KIND, int ORDER, bool PA, class SER>
+MF_DEV void gmf_lane(const double* a, double al, const double* __restrict__ bt, GmfOut& o,
+                     const double* __restrict__ rinv, const SER& series) {
+  constexpr int ND = ORDER == 0 ? 3 : (ORDER == 1 ? 9 : 19);
+  GmfNodes st;
+  GmfSeriesIn sin;
+  gmf_nodes<KIND, ORDER, PA>(a, al, bt, rinv, st, sin);
+  if constexpr (SER::K_ONLY) {
+    o.K = sin.K;
+    series.post(sin, st);
+    return;
+  }
+  cpx acc[ND];
+  series(sin, acc);
+  gmf_finish<KIND, ORDER, PA>(a, al, st, acc, o);
 }
 
 // ---- assembly of J and H from the staged per-matrix results of one matrix: r1, r2 (9 each, row-major; r_0 = I is

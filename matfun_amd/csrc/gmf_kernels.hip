@@ -23,11 +23,12 @@ constexpr int gmf_wg_lds() {
 }
 
 // Register budgets: FP32 outputs at order 1 four wavefronts per SIMD (128 VGPRs; the LDS of 8.9 KB per tile, or
-// 36.9 KB per workgroup of four, allows sixteen per CU); order 2 with per-matrix exponents two (256 VGPRs: left to
-// itself the compiler takes a few more and drops to one)
+// 36.9 KB per workgroup of four, allows sixteen per CU); order 2 two (256 registers in all: left to itself the
+// compiler may add a few accumulation registers to its 256 VGPRs, which leaves one wavefront per SIMD -- measured 45%
+// slower)
 template <int KIND, int ORDER, class OT, bool PA>
 __global__ void __launch_bounds__((WAVE * gmf_nw<KIND, ORDER, OT>()))
-    __attribute__((amdgpu_waves_per_eu((ORDER == 1 && sizeof(OT) == 4) ? 4 : ((ORDER == 2 && PA) ? 2 : 1))))
+    __attribute__((amdgpu_waves_per_eu((ORDER == 1 && sizeof(OT) == 4) ? 4 : (ORDER == 2 ? 2 : 1))))
     gmf_kernel(int64_t n, const double* __restrict__ A, OT* __restrict__ F, OT* __restrict__ J, OT* __restrict__ H,
                double al, const double* __restrict__ alv, GmfCoef bt) {
   constexpr int TB = (GmfLds<ORDER, OT>::BYTES + 15) / 16 * 16, NW = gmf_nw<KIND, ORDER, OT>();

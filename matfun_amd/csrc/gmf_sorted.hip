@@ -16,10 +16,12 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <atomic>
 #include <mutex>
 
 #include "gmf_sorted.hpp"
 #include "gmf_tile.hpp"
+#include "smf_launch.hpp"
 
 namespace mf {
 
@@ -46,6 +48,54 @@ __global__ void __launch_bounds__(KLEN_THREADS)
   GmfOut o;
   gmf_lane<KIND, ORDER, PA>(a, al, bt.b, o, g_gmf_recip.r, GmfSeriesLength{});
   key[i] = (uint8_t)(GMF_KMAX - o.K);
+}
+
+// the series inputs and the key of matrices [0, n) (one lane per matrix; the first pass of the staged evaluation)
+template <int KIND, int ORDER, bool PA>
+__global__ void __launch_bounds__(KLEN_THREADS)
+    gmf_prep_kernel(int64_t n, const double* __restrict__ A, double al, const double* __restrict__ alv, GmfCoef bt,
+                    uint8_t* __restrict__ key, GmfSeriesIn* __restrict__ sin, GmfNodes* __restrict__ nodes) {
+  const int64_t i = (int64_t)blockIdx.x * KLEN_THREADS + threadIdx.x;
+  if (i >= n) return;
+  double a[9];
+#pragma unroll
+  for (int k = 0; k < 9; k++) a[k] = A[9 * i + k];
+  if (PA) {
+    al = alv[i];
+    al = fabs(al) <= 1.7976931348623157e308 ? al : 1.0;
+  }
+  GmfOut o;
+  gmf_lane<KIND, ORDER, PA>(a, al, bt.b, o, g_gmf_recip.r, GmfSeriesLength{sin + i, nodes + i});
+  key[i] = (uint8_t)(GMF_KMAX - o.K);
+}
+
+// the group series of the matrices in sorted order: lane t of the batch takes matrix perm[t] (its inputs from
+// sin, its ND sums to sums at the matrix's own row)
+template <int KIND, int ORDER, bool PA>
+__global__ void __launch_bounds__(KLEN_THREADS)
+    gmf_series_kernel(int64_t n, const int32_t* __restrict__ perm, const GmfSeriesIn* __restrict__ sin, GmfCoef bt,
+                      cpx* __restrict__ sums) {
+  constexpr int ND = ORDER == 0 ? 3 : (ORDER == 1 ? 9 : 19);
+  const int64_t t = (int64_t)blockIdx.x * KLEN_THREADS + threadIdx.x;
+  if (t >= n) return;
+  const int64_t i = perm[t];
+  cpx acc[ND];
+  gmf_series<KIND, ORDER, PA>(sin[i], bt.b, acc);
+#pragma unroll
+  for (int c = 0; c < ND; c++) sums[i * ND + c] = acc[c];
+}
+
+// the tile body of the homogeneous kernel over consecutive matrices, each finished from its staged nodes and sums
+template <int KIND, int ORDER, class OT, bool PA>
+__global__ void __launch_bounds__(WAVE) __attribute__((amdgpu_waves_per_eu(ORDER == 2 ? 2 : 1)))
+    gmf_finish_kernel(int64_t n, const double* __restrict__ A, OT* __restrict__ F, OT* __restrict__ J,
+                      OT* __restrict__ H, double al, const double* __restrict__ alv, GmfCoef bt, GmfStaged staged) {
+  __shared__ __attribute__((aligned(16))) unsigned char lds[GmfLds<ORDER, OT>::BYTES];
+  const int lane = threadIdx.x;
+  const int64_t base = (int64_t)blockIdx.x * WAVE, left = n - base;
+  const int nm = left < WAVE ? (int)left : WAVE;
+  gmf_tile<KIND, ORDER, OT, PA, RunRows, 1, true>(RunRows{base}, nm, A, F, J, H, al, alv, bt.b, lds, lane, nullptr, 0,
+                                                  staged);
 }
 
 // hist[c][b]: how many of the keys in chunk c (matrices [c chunk, (c + 1) chunk)) equal b
@@ -108,14 +158,14 @@ __global__ void __launch_bounds__(SORT_THREADS)
 }
 
 // the homogeneous kernel's tile body over perm[64 t .. 64 t + 63]; register budgets as gmf_kernel's one-tile
-// workgroups (order 2 with per-matrix exponents: two wavefronts per SIMD).  Workgroup b takes tile t = b step mod T
+// workgroups (order 2: two wavefronts per SIMD).  Workgroup b takes tile t = b step mod T
 // (T tiles, step coprime to T, about T / SORT_MIX): the workgroups resident at one time hold tiles from every part
 // of the length order -- long series (compute) next to short ones (stores) -- instead of the longest ones together
 // and then the shortest (measured on MI355X against the plain order: 1-6% faster at order 2; SORT_MIX 16 or 64
 // gained less)
 constexpr int64_t SORT_MIX = 4;
 template <int KIND, int ORDER, class OT, bool PA>
-__global__ void __launch_bounds__(WAVE) __attribute__((amdgpu_waves_per_eu((ORDER == 2 && PA) ? 2 : 1)))
+__global__ void __launch_bounds__(WAVE) __attribute__((amdgpu_waves_per_eu(ORDER == 2 ? 2 : 1)))
     gmf_perm_kernel(int64_t n, int64_t step, const int32_t* __restrict__ perm, const double* __restrict__ A,
                     OT* __restrict__ F, OT* __restrict__ J, OT* __restrict__ H, double al,
                     const double* __restrict__ alv, GmfCoef bt) {
@@ -202,41 +252,158 @@ void sorted_part(hipStream_t st, int order, int64_t m, const double* A, OT* F, O
   perm_launch<KIND, OT, PA>(st, m, s.perm, A, F, J, H, al, alv, bt);
 }
 
-}  // namespace
-
-size_t gmf_sort_workspace(int64_t n) {
-  if (n < 0) return 0;
-  return align256((size_t)n) + align256(4 * (size_t)n) + 2 * align256(4 * (size_t)KSORT_MAXCH * KBINS);
+// the staged evaluation of m matrices at order `order`: series inputs and keys (natural order), the sort, the series
+// in sorted order, then the tile body in natural order with the sums (contiguous stores).  Workspace: staged_bytes.
+size_t staged_bytes(int64_t m, int order) {
+  const int nd = order == 0 ? 3 : (order == 1 ? 9 : 19);
+  return align256((size_t)m) + align256(4 * (size_t)m) + 2 * align256(4 * (size_t)KSORT_MAXCH * KBINS) +
+         align256(sizeof(GmfSeriesIn) * (size_t)m) + align256(sizeof(GmfNodes) * (size_t)m) +
+         align256(sizeof(cpx) * nd * (size_t)m);
 }
 
-int64_t gmf_sort_capacity(size_t bytes) {
+template <int KIND, int ORDER, class OT, bool PA>
+void staged_order(hipStream_t st, int64_t m, const double* A, OT* F, OT* J, OT* H, double al, const double* alv,
+                  const GmfCoef& bt, void* work) {
+  const SortSpace s = carve(work, m);
+  unsigned char* p = reinterpret_cast<unsigned char*>(s.off) + align256(4 * (size_t)KSORT_MAXCH * KBINS);
+  GmfSeriesIn* sin = reinterpret_cast<GmfSeriesIn*>(p);
+  p += align256(sizeof(GmfSeriesIn) * (size_t)m);
+  GmfNodes* nodes = reinterpret_cast<GmfNodes*>(p);
+  p += align256(sizeof(GmfNodes) * (size_t)m);
+  cpx* sums = reinterpret_cast<cpx*>(p);
+  int64_t chunk = (m + KSORT_MAXCH - 1) / KSORT_MAXCH;
+  chunk = (chunk + SORT_THREADS - 1) / SORT_THREADS * SORT_THREADS;
+  const int nch = (int)((m + chunk - 1) / chunk);
+  const unsigned gl = (unsigned)((m + KLEN_THREADS - 1) / KLEN_THREADS);
+  hipLaunchKernelGGL((gmf_prep_kernel<KIND, ORDER, PA>), dim3(gl), dim3(KLEN_THREADS), 0, st, m, A, al, alv, bt, s.key,
+                     sin, nodes);
+  hipLaunchKernelGGL(key_hist_kernel, dim3(nch), dim3(SORT_THREADS), 0, st, m, chunk, s.key, s.hist);
+  hipLaunchKernelGGL(key_scan_kernel, dim3(1), dim3(SORT_THREADS), 0, st, nch, s.hist, s.off);
+  hipLaunchKernelGGL(key_scatter_kernel, dim3(nch), dim3(SORT_THREADS), 0, st, m, chunk, s.key, s.off, s.perm);
+  hipLaunchKernelGGL((gmf_series_kernel<KIND, ORDER, PA>), dim3(gl), dim3(KLEN_THREADS), 0, st, m, s.perm, sin, bt,
+                     sums);
+  hipLaunchKernelGGL((gmf_finish_kernel<KIND, ORDER, OT, PA>), dim3((unsigned)((m + WAVE - 1) / WAVE)), dim3(WAVE), 0,
+                     st, m, A, F, J, H, al, alv, bt, GmfStaged{nodes, sums});
+}
+
+template <int KIND, class OT, bool PA>
+void staged_part(hipStream_t st, int order, int64_t m, const double* A, OT* F, OT* J, OT* H, double al,
+                 const double* alv, const GmfCoef& bt, void* work) {
+  if (order == 2) staged_order<KIND, 2, OT, PA>(st, m, A, F, J, H, al, alv, bt, work);
+  else if (order == 1) staged_order<KIND, 1, OT, PA>(st, m, A, F, J, H, al, alv, bt, work);
+  else staged_order<KIND, 0, OT, PA>(st, m, A, F, J, H, al, alv, bt, work);
+}
+
+size_t perm_bytes(int64_t m) {
+  return align256((size_t)m) + align256(4 * (size_t)m) + 2 * align256(4 * (size_t)KSORT_MAXCH * KBINS);
+}
+
+// the most matrices a workspace of `bytes` takes in one part with the given layout (per-matrix bytes `per`, plus
+// the fixed tables), for the byte count function `need`
+template <class Need>
+int64_t capacity_of(size_t bytes, size_t per, Need need) {
   const size_t fixed = 2 * align256(4 * (size_t)KSORT_MAXCH * KBINS);
   if (bytes < fixed) return 0;
-  int64_t m = (int64_t)((bytes - fixed) / 5);
-  while (m > 0 && gmf_sort_workspace(m) > bytes) m--;       // (at most ~100 steps: the alignment padding)
+  int64_t m = (int64_t)((bytes - fixed) / per);
+  while (m > 0 && need(m) > bytes) m--;                     // (a few steps: the alignment padding)
   return m;
 }
 
+// the strategy table: [function slot: exp, log, pow, sqrt, per-matrix exponents][order][precision]
+std::atomic<int> g_strategy[5][3][2];
+std::once_flag g_strategy_once;
+
+// defaults measured on MI355X (general mix, 2^20 matrices): short series (exp) and order 0 gain nothing from the
+// length order (its passes cost more than the shorter loops save); at order 1 the staged evaluation is 20% faster
+// for log / powers, at order 2 for the mixed form 14%, while the FP64 order-2 kernel, bound by its stores with the
+// series hidden behind them, is as fast unsorted
+int default_strategy(int slot, int order, int mixed) {
+  if (slot == 0 || order == 0) return GMF_SORT_PLAIN;
+  if (order == 1) return GMF_SORT_STAGED;
+  return mixed ? GMF_SORT_STAGED : GMF_SORT_PLAIN;
+}
+
+void init_strategies() {
+  std::call_once(g_strategy_once, [] {
+    for (int k = 0; k < 5; k++)
+      for (int o = 0; o < 3; o++)
+        for (int x = 0; x < 2; x++) g_strategy[k][o][x].store(default_strategy(k, o, x));
+  });
+}
+
+constexpr int64_t STAGED_MIN_PART = (int64_t)1 << 15;   // (smaller parts would leave the GPU half idle per pass)
+
+void plain_launch(hipStream_t st, int kind, int64_t n, const double* A, double* F, double* J, double* H, double al,
+                  const double* alv, const GmfCoef& bt) {
+  if (alv) gmf_launch_pow(st, n, alv, A, F, J, H, bt);
+  else gmf_launch(st, kind, n, A, F, J, H, al, bt);
+}
+void plain_launch(hipStream_t st, int kind, int64_t n, const double* A, float* F, float* J, float* H, double al,
+                  const double* alv, const GmfCoef& bt) {
+  if (alv) gmf_launch_pow_mixed(st, n, alv, A, F, J, H, bt);
+  else gmf_launch_mixed(st, kind, n, A, F, J, H, al, bt);
+}
+
+}  // namespace
+
+int gmf_sort_strategy_get(int slot, int order, bool mixed) {
+  init_strategies();
+  return g_strategy[slot][order][mixed ? 1 : 0].load();
+}
+void gmf_sort_strategy_set(int slot, int order, bool mixed, int strategy) {
+  init_strategies();
+  g_strategy[slot][order][mixed ? 1 : 0].store(strategy);
+}
+
+size_t gmf_sort_workspace(int64_t n) {
+  if (n < 0) return 0;
+  const size_t a = staged_bytes(n, 2), b = perm_bytes(n);
+  return a > b ? a : b;
+}
+
+int64_t gmf_sort_capacity(size_t bytes) { return capacity_of(bytes, 5, perm_bytes); }
+
 template <class OT>
-int gmf_sorted_launch(hipStream_t st, int kind, int order, int64_t n, const double* A, OT* F, OT* J, OT* H, double al,
-                      const double* alv, const GmfCoef& bt, void* work, int64_t cap) {
+int gmf_sorted_launch(hipStream_t st, int slot, int kind, int order, int64_t n, const double* A, OT* F, OT* J, OT* H,
+                      double al, const double* alv, const GmfCoef& bt, void* work, size_t bytes) {
+  int strat = gmf_sort_strategy_get(slot, order, sizeof(OT) == 4);
+  int64_t cap = 0;
+  if (strat == GMF_SORT_STAGED) {
+    const int nd = order == 0 ? 3 : (order == 1 ? 9 : 19);
+    cap = capacity_of(bytes, 5 + sizeof(GmfSeriesIn) + sizeof(GmfNodes) + nd * sizeof(cpx),
+                      [order](int64_t m) { return staged_bytes(m, order); });
+    if (cap < (n < STAGED_MIN_PART ? n : STAGED_MIN_PART)) strat = GMF_SORT_PERM;   // (too little room: permuted)
+  }
+  if (strat == GMF_SORT_PERM) cap = gmf_sort_capacity(bytes);
+  if (strat == GMF_SORT_PLAIN) cap = MAX_PER_LAUNCH;
+  cap = cap < MAX_PER_LAUNCH ? cap : MAX_PER_LAUNCH;            // (32-bit permutation entries, grids below 2^32)
+  if (cap <= 0) return -1;
   for (int64_t lo = 0; lo < n; lo += cap) {
     const int64_t m = n - lo < cap ? n - lo : cap;
     const double* Ap = A + 9 * lo;
     OT *Fp = F + 9 * lo, *Jp = J ? J + 81 * lo : nullptr, *Hp = H ? H + 729 * lo : nullptr;
     const double* ap = alv ? alv + lo : nullptr;
-    if (alv) sorted_part<POW, OT, true>(st, order, m, Ap, Fp, Jp, Hp, 1.0, ap, bt, work);
-    else if (kind == EXP) sorted_part<EXP, OT, false>(st, order, m, Ap, Fp, Jp, Hp, al, nullptr, bt, work);
-    else if (kind == LOG) sorted_part<LOG, OT, false>(st, order, m, Ap, Fp, Jp, Hp, al, nullptr, bt, work);
-    else sorted_part<POW, OT, false>(st, order, m, Ap, Fp, Jp, Hp, al, nullptr, bt, work);
+    if (strat == GMF_SORT_PLAIN) {
+      plain_launch(st, kind, m, Ap, Fp, Jp, Hp, al, ap, bt);
+    } else if (strat == GMF_SORT_STAGED) {
+      if (alv) staged_part<POW, OT, true>(st, order, m, Ap, Fp, Jp, Hp, 1.0, ap, bt, work);
+      else if (kind == EXP) staged_part<EXP, OT, false>(st, order, m, Ap, Fp, Jp, Hp, al, nullptr, bt, work);
+      else if (kind == LOG) staged_part<LOG, OT, false>(st, order, m, Ap, Fp, Jp, Hp, al, nullptr, bt, work);
+      else staged_part<POW, OT, false>(st, order, m, Ap, Fp, Jp, Hp, al, nullptr, bt, work);
+    } else {
+      if (alv) sorted_part<POW, OT, true>(st, order, m, Ap, Fp, Jp, Hp, 1.0, ap, bt, work);
+      else if (kind == EXP) sorted_part<EXP, OT, false>(st, order, m, Ap, Fp, Jp, Hp, al, nullptr, bt, work);
+      else if (kind == LOG) sorted_part<LOG, OT, false>(st, order, m, Ap, Fp, Jp, Hp, al, nullptr, bt, work);
+      else sorted_part<POW, OT, false>(st, order, m, Ap, Fp, Jp, Hp, al, nullptr, bt, work);
+    }
     if (hipGetLastError() != hipSuccess) return -2;
   }
   return 0;
 }
-template int gmf_sorted_launch<double>(hipStream_t, int, int, int64_t, const double*, double*, double*, double*,
-                                       double, const double*, const GmfCoef&, void*, int64_t);
-template int gmf_sorted_launch<float>(hipStream_t, int, int, int64_t, const double*, float*, float*, float*, double,
-                                      const double*, const GmfCoef&, void*, int64_t);
+template int gmf_sorted_launch<double>(hipStream_t, int, int, int, int64_t, const double*, double*, double*, double*,
+                                       double, const double*, const GmfCoef&, void*, size_t);
+template int gmf_sorted_launch<float>(hipStream_t, int, int, int, int64_t, const double*, float*, float*, float*,
+                                      double, const double*, const GmfCoef&, void*, size_t);
 
 }  // namespace mf
 
@@ -278,6 +445,7 @@ extern "C" int gmf_hip_eval_sorted_noalloc(int kind, double alpha, int64_t n, co
   using namespace mf;
   if (kind < EXP || kind > SQRT || n < 0 || (H && !J) || (kind == POW && !(alpha > 0))) return -1;
   if (n > STATIC_MAX_N) return -1;                        // (beyond the static workspace: split the batch)
+  const int slot = kind;
   if (kind == SQRT) kind = POW, alpha = 0.5;
   if (kind != POW) alpha = 0.0;
   if (n == 0) return 0;
@@ -286,6 +454,6 @@ extern "C" int gmf_hip_eval_sorted_noalloc(int kind, double alpha, int64_t n, co
   GmfCoef bt;
   gmf_coefs(kind, alpha, bt);
   const int order = H ? 2 : (J ? 1 : 0);
-  return gmf_sorted_launch<double>((hipStream_t)stream, kind, order, n, A, F, J, H, alpha, nullptr, bt, work,
-                                   STATIC_MAX_N);
+  return gmf_sorted_launch<double>((hipStream_t)stream, slot, kind, order, n, A, F, J, H, alpha, nullptr, bt, work,
+                                   STATIC_BYTES);
 }

@@ -11,14 +11,26 @@ struct GmfCoef;
 constexpr int KBINS = 201;           // keys 200 - K for series lengths K = 0 .. 200
 constexpr int KSORT_MAXCH = 128;     // histogram chunks of one part at most
 
-// workspace bytes for n matrices in one part; the number of matrices a workspace of `bytes` takes in one part
+// evaluation strategies of the workspace entry points (gmf_hip_eval_sorted*), per function slot (0 exp, 1 log, 2 pow,
+// 3 sqrt, 4 per-matrix exponents), order and precision:
+//   PLAIN  the homogeneous kernel in input order (gmf_hip_eval; no workspace used)
+//   PERM   keys, sort, the tile body over the length-sorted permutation (bitwise gmf_hip_eval's results)
+//   STAGED nodes and series inputs per matrix, the series in length order, the rest in input order (contiguous
+//          stores; equal to gmf_hip_eval's results up to rounding)
+constexpr int GMF_SORT_PLAIN = 0, GMF_SORT_PERM = 1, GMF_SORT_STAGED = 2;
+int gmf_sort_strategy_get(int slot, int order, bool mixed);
+void gmf_sort_strategy_set(int slot, int order, bool mixed, int strategy);
+
+// workspace bytes for n matrices in one part (every strategy and order); the number of matrices a workspace of
+// `bytes` takes in one part of the permuted evaluation (the least room a workspace must offer)
 size_t gmf_sort_workspace(int64_t n);
 int64_t gmf_sort_capacity(size_t bytes);
 
-// the batch in parts of at most cap matrices (kind normalised as gmf_hip_eval; alv: per-matrix exponents or NULL;
-// order from J / H as the C ABI).  Returns 0 or -2 (launch error).
+// the batch through the slot's strategy, in parts of what the workspace holds (the staged evaluation falls back to
+// the permuted one when the workspace holds fewer than 2^15 matrices of it); kind normalised as gmf_hip_eval; alv:
+// per-matrix exponents or NULL; order from J / H as the C ABI.  Returns 0, -1 (no room) or -2 (launch error).
 template <class OT>
-int gmf_sorted_launch(hipStream_t st, int kind, int order, int64_t n, const double* A, OT* F, OT* J, OT* H, double al,
-                      const double* alv, const GmfCoef& bt, void* work, int64_t cap);
+int gmf_sorted_launch(hipStream_t st, int slot, int kind, int order, int64_t n, const double* A, OT* F, OT* J, OT* H,
+                      double al, const double* alv, const GmfCoef& bt, void* work, size_t bytes);
 
 }  // namespace mf

@@ -102,10 +102,19 @@ struct GmfLds {
 // its own series; NW > 1: the tile is one of NW in a workgroup of NW wavefronts (every one calling gmf_tile once) that
 // share out the series (GmfSeriesShared; pool: its LDS, tid = threadIdx.x).  Ends with the tile's outputs written and
 // a wave barrier (the LDS may be reused).
-template <int KIND, int ORDER, class OT, bool PA, class R, int NW = 1>
+// the nodes and group series sums of each matrix computed by earlier passes (the staged evaluation, gmf_sorted.hip):
+// nodes[row] and sums[row ND .. row ND + ND - 1]
+struct GmfStaged {
+  const GmfNodes* nodes;
+  const cpx* sums;
+};
+
+// STAGED: each lane finishes its matrix from staged nodes and sums (gmf_finish) instead of running gmf_lane
+template <int KIND, int ORDER, class OT, bool PA, class R, int NW = 1, bool STAGED = false>
 MF_DEV void gmf_tile(const R& rows, int nm, const double* __restrict__ A, OT* __restrict__ F, OT* __restrict__ J,
                      OT* __restrict__ H, double al, const double* __restrict__ alv, const double* __restrict__ bt,
-                     unsigned char* lds, int lane, unsigned char* pool = nullptr, int tid = 0) {
+                     unsigned char* lds, int lane, unsigned char* pool = nullptr, int tid = 0,
+                     GmfStaged staged = GmfStaged{nullptr, nullptr}) {
   using L = GmfLds<ORDER, OT>;
   constexpr int VEC = 16 / (int)sizeof(OT);             // (elements per 16-byte store)
   double* xin = reinterpret_cast<double*>(lds);
@@ -127,7 +136,15 @@ MF_DEV void gmf_tile(const R& rows, int nm, const double* __restrict__ A, OT* __
 
   // ---- phase 1
   GmfOut o;
-  if constexpr (NW > 1)
+  if constexpr (STAGED) {
+    constexpr int ND = ORDER == 0 ? 3 : (ORDER == 1 ? 9 : 19);
+    const int64_t row = rows.m(lane < nm ? lane : 0);
+    const GmfNodes st = staged.nodes[row];
+    cpx acc[ND];
+#pragma unroll
+    for (int c = 0; c < ND; c++) acc[c] = staged.sums[row * ND + c];
+    gmf_finish<KIND, ORDER, PA>(a, al, st, acc, o);
+  } else if constexpr (NW > 1)
     gmf_lane<KIND, ORDER, PA>(a, al, bt, o, g_gmf_recip.r, GmfSeriesShared<KIND, ORDER, PA, NW>{bt, pool, tid});
   else
     gmf_lane<KIND, ORDER, PA>(a, al, bt, o, g_gmf_recip.r, GmfSeriesLocal<KIND, ORDER, PA>{bt});

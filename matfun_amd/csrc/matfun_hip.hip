@@ -103,24 +103,24 @@ int gmf_call(int kind, double alpha, int64_t n, const double* A, OT* F, OT* J, O
 // general matrices ordered by series length through the caller's workspace (gmf_sorted.hip); the same checks and
 // normalisation as gmf_call / gmf_pow_call (alv: per-matrix exponents, else kind and alpha), plus -1 for a workspace
 // that holds fewer than min(n, 64) matrices
-constexpr int64_t SORT_PART_MAX = (int64_t)1 << 30;      // matrices per part (32-bit permutation entries)
 template <class OT>
 int gmf_sorted_call(int kind, double alpha, const double* alv, int64_t n, const double* A, OT* F, OT* J, OT* H,
                     void* work, size_t work_bytes, void* stream) {
   if (alv) kind = mf::POW, alpha = 1.0;
   if (!valid(kind, alpha, n, J, H)) return -1;
+  const int slot = alv ? 4 : kind;
   if (kind == mf::SQRT) kind = mf::POW, alpha = 0.5;
   if (kind != mf::POW) alpha = 0.0;
   if (n == 0) return 0;
-  int64_t cap = mf::gmf_sort_capacity(work ? work_bytes : 0);
+  const int64_t cap = mf::gmf_sort_capacity(work ? work_bytes : 0);
   if (cap < (n < 64 ? n : 64)) return -1;
   if (n > GMF_MAX_N) return -2;
-  cap = cap < SORT_PART_MAX ? cap : SORT_PART_MAX;
   mf::GmfCoef bt;
   if (alv) mf::gmf_recips(bt);
   else mf::gmf_coefs(kind, alpha, bt);
   const int order = H ? 2 : (J ? 1 : 0);
-  return mf::gmf_sorted_launch<OT>((hipStream_t)stream, kind, order, n, A, F, J, H, alpha, alv, bt, work, cap);
+  return mf::gmf_sorted_launch<OT>((hipStream_t)stream, slot, kind, order, n, A, F, J, H, alpha, alv, bt, work,
+                                   work_bytes);
 }
 
 void packed_chunk(hipStream_t st, int kind, int64_t n, const double* x, double* F, double* J, double* H, double al,
@@ -276,6 +276,19 @@ extern "C" int gmf_hip_eval_pow_sorted_mixed(const double* alpha, int64_t n, con
                                              float* H, void* work, size_t work_bytes, void* stream) {
   if (n > 0 && !alpha) return -1;
   return gmf_sorted_call(mf::POW, 1.0, n > 0 ? alpha : nullptr, n, A, F, J, H, work, work_bytes, stream);
+}
+
+extern "C" int gmf_hip_set_sort_strategy(int kind, int order, int mixed, int strategy) {
+  if (kind < mf::EXP || kind > 4 || order < 0 || order > 2 || strategy < mf::GMF_SORT_PLAIN ||
+      strategy > mf::GMF_SORT_STAGED)
+    return -1;
+  mf::gmf_sort_strategy_set(kind, order, mixed != 0, strategy);
+  return 0;
+}
+
+extern "C" int gmf_hip_get_sort_strategy(int kind, int order, int mixed) {
+  if (kind < mf::EXP || kind > 4 || order < 0 || order > 2) return -1;
+  return mf::gmf_sort_strategy_get(kind, order, mixed != 0);
 }
 
 extern "C" int smf_hip_set_tile(int kind, int matrix_layout, int mixed, int hpp) {

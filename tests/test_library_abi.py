@@ -125,7 +125,7 @@ def test_sorted_entry_points_reject_before_device_work():
     lib = ctypes.CDLL(LIB)
     lib.gmf_hip_sort_workspace.argtypes, lib.gmf_hip_sort_workspace.restype = [ctypes.c_int64], ctypes.c_size_t
     ws = lib.gmf_hip_sort_workspace
-    assert ws(-1) == 0 and ws(64) < ws(65) < ws(1 << 20) and ws(1 << 20) < 6 * (1 << 20)
+    assert ws(-1) == 0 and ws(64) < ws(65) < ws(1 << 20) and ws(1 << 20) < 600 * (1 << 20)
     dummy = ctypes.c_void_p(0x1000)
     big = 1 << 40
     args = [ctypes.c_int, ctypes.c_double, ctypes.c_int64] + [ctypes.c_void_p] * 5 + [ctypes.c_size_t, ctypes.c_void_p]
@@ -136,7 +136,7 @@ def test_sorted_entry_points_reject_before_device_work():
         assert f(0, 0.0, 10, dummy, dummy, None, dummy, dummy, big, None) == -1          # H without J
         assert f(0, 0.0, -1, dummy, dummy, None, None, dummy, big, None) == -1           # n < 0
         assert f(0, 0.0, 10, dummy, dummy, None, None, None, big, None) == -1            # no workspace
-        assert f(0, 0.0, 10, dummy, dummy, None, None, dummy, ws(64) - 1, None) == -1    # holds fewer than 10
+        assert f(0, 0.0, 10, dummy, dummy, None, None, dummy, ws(0), None) == -1         # holds no matrix
         assert f(0, 0.0, 0, None, None, None, None, None, 0, None) == 0                  # n == 0
         assert f(0, 0.0, (2 ** 31 - 1) * 64 + 1, dummy, dummy, None, None, dummy, big, None) == -2
     pargs = [ctypes.c_void_p, ctypes.c_int64] + [ctypes.c_void_p] * 5 + [ctypes.c_size_t, ctypes.c_void_p]
@@ -144,15 +144,15 @@ def test_sorted_entry_points_reject_before_device_work():
         f.argtypes, f.restype = pargs, ctypes.c_int
         assert f(None, 10, dummy, dummy, None, None, dummy, big, None) == -1             # no exponents
         assert f(dummy, 10, dummy, dummy, None, dummy, dummy, big, None) == -1           # H without J
-        assert f(dummy, 100, dummy, dummy, None, None, dummy, ws(64) - 1, None) == -1    # workspace below 64
+        assert f(dummy, 100, dummy, dummy, None, None, dummy, ws(0), None) == -1         # holds no matrix
         assert f(None, 0, None, None, None, None, None, 0, None) == 0
 
 
 @pytest.mark.skipif(not os.path.exists(LIB), reason="HIP library not built (python -m matfun_amd.build)")
 def test_sorted_workspace_of_n_always_suffices():
-    """the workspace rule at its boundary: one byte less than gmf_hip_sort_workspace(min(n, 64)) is refused for every
-    n (only rejections are called here: dummy pointers must never reach a launch; the GPU tests run the accepted
-    sizes, tests/test_gpu_sorted.py)"""
+    """the workspace rule: the fixed tables alone (gmf_hip_sort_workspace(0)) are refused for every n (only rejections
+    are called here: dummy pointers must never reach a launch; the GPU tests run the accepted sizes,
+    tests/test_gpu_sorted.py)"""
     lib = ctypes.CDLL(LIB)
     lib.gmf_hip_sort_workspace.argtypes, lib.gmf_hip_sort_workspace.restype = [ctypes.c_int64], ctypes.c_size_t
     f = lib.gmf_hip_eval_sorted
@@ -160,5 +160,5 @@ def test_sorted_workspace_of_n_always_suffices():
                                                                                             ctypes.c_void_p]
     dummy = ctypes.c_void_p(0x1000)
     for n in (1, 2, 63, 64, 65, 1000):
-        small = lib.gmf_hip_sort_workspace(min(n, 64)) - 1
+        small = lib.gmf_hip_sort_workspace(0) + 4 * min(n, 64) - 8         # (room for fewer than min(n, 64))
         assert f(0, 0.0, n, dummy, dummy, None, None, dummy, small, None) == -1
