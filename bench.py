@@ -335,13 +335,20 @@ def main():
     torch.cuda.set_device(dev)
     lib = matfun_amd.library()
     n, ng = args.n, args.gmf_n
-    tiles = None
+    tiles = gtiles = None
     if not args.no_tune:
         import matfun_amd.tune as T
         log("bench: tuning the symmetric order-2 H store tile (matfun_amd.tune)")
         tuned = T.tune_smf_tiles(layouts=("matrix",))
         tiles = {f"{k[0]} {k[1]} {k[2]}": v[0] for k, v in tuned.items()}
         log(f"bench: tiles {tiles}")
+        log("bench: tuning the general-matrix workgroup shape and evaluation strategy (matfun_amd.tune)")
+        gt = T.tune_gmf(orders=(0, 1, 2), n=args.gmf_n)
+        gtiles = {f"{k[0]} o{k[1]} {k[2]}": {"tiles": v["tiles"][0], "strategy": v["strategy"][0],
+                                             "ms": {f"tiles{a}": round(b, 4) for a, b in v["tiles"][1].items()} |
+                                             {f"strategy{a}": round(b, 4) for a, b in v["strategy"][1].items()}}
+                  for k, v in gt.items()}
+        log("bench: gmf " + ", ".join(f"{k}: {v['tiles']}/{v['strategy']}" for k, v in gtiles.items()))
     log(f"bench: generating {n} symmetric and {ng} general inputs (seed 0)")
     A = W.spd(n, seed=0)
     dA = torch.from_numpy(A).to(dev)
@@ -576,7 +583,7 @@ def main():
            "roofline": {"bound": "hbm", "achieved": head["achieved_GBs"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
                         "frac": head["frac"], "traffic": profiled_traffic(n), "kernel": "mf::smf_kernel<KIND,true,2,double>",
                         "avg_kernel_ms": head["avg_kernel_ms"], "measured_copy_GBs": copy_gbs},
-           "legs": legs, "tuned_tiles": tiles}
+           "legs": legs, "tuned_tiles": tiles, "tuned_gmf": gtiles}
     if not (args.only_headline or args.only_hencky):
         out["mixed_vs_fp64"] = legs_c
         out["hetero_batch"] = legs_b

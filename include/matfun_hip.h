@@ -231,6 +231,16 @@ int gmf_hip_hencky(int64_t n, const double *A, double *L, double *psi, double *g
 int smf_hip_set_tile(int kind, int matrix_layout, int mixed, int hpp);
 int smf_hip_get_tile(int kind, int matrix_layout, int mixed);
 
+/* ---- workgroup shape of the general-matrix kernels (gmf_hip_eval*, gmf_hip_eval_pow*, and strategy 0 of the sorted
+ * entry points; no counterpart in the reference), per function (kind 0..3; 4 = per-matrix exponents), order (0..2)
+ * and precision (mixed 0 FP64, 1 FP32 outputs), for the whole process: tiles = 1, 2 or 4 wavefronts of 64 matrices
+ * per workgroup; with 2 or 4 the wavefronts share out their group series by length (each lane then runs a series of
+ * about the length of its neighbours').  Results do not depend on it.  Defaults measured on MI355X (4 for the FP32
+ * order-1 kernels but exp's and the order-0 log / pow / sqrt kernels, 1 elsewhere); matfun_amd.tune picks them by
+ * timing.  Returns 0 / -1 (set), the choice or -1 (get). */
+int gmf_hip_set_tiles(int kind, int order, int mixed, int tiles);
+int gmf_hip_get_tiles(int kind, int order, int mixed);
+
 /* library version ("0.1.3"-compatible API level plus this build); replaces smf_version (c/include/matfun/matfun.h:53) */
 const char *smf_hip_version(void);
 

@@ -62,6 +62,8 @@ def library():
         lib.gmf_hip_eval_sorted_noalloc.argtypes, lib.gmf_hip_eval_sorted_noalloc.restype = args, ctypes.c_int
         lib.gmf_hip_set_sort_strategy.argtypes = [ctypes.c_int] * 4
         lib.gmf_hip_get_sort_strategy.argtypes = [ctypes.c_int] * 3
+        lib.gmf_hip_set_tiles.argtypes = [ctypes.c_int] * 4
+        lib.gmf_hip_get_tiles.argtypes = [ctypes.c_int] * 3
         for fn in (lib.mf_hip_hencky, lib.mf_hip_hencky_mixed):
             fn.argtypes, fn.restype = [ctypes.c_int64] + [ctypes.c_void_p] * 7, ctypes.c_int
         for fn in (lib.smf_hip_hencky_matrix, lib.smf_hip_hencky, lib.gmf_hip_hencky):

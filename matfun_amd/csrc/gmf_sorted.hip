@@ -333,15 +333,15 @@ void init_strategies() {
 
 constexpr int64_t STAGED_MIN_PART = (int64_t)1 << 15;   // (smaller parts would leave the GPU half idle per pass)
 
-void plain_launch(hipStream_t st, int kind, int64_t n, const double* A, double* F, double* J, double* H, double al,
-                  const double* alv, const GmfCoef& bt) {
+void plain_launch(hipStream_t st, int slot, int kind, int64_t n, const double* A, double* F, double* J, double* H,
+                  double al, const double* alv, const GmfCoef& bt) {
   if (alv) gmf_launch_pow(st, n, alv, A, F, J, H, bt);
-  else gmf_launch(st, kind, n, A, F, J, H, al, bt);
+  else gmf_launch(st, slot, kind, n, A, F, J, H, al, bt);
 }
-void plain_launch(hipStream_t st, int kind, int64_t n, const double* A, float* F, float* J, float* H, double al,
-                  const double* alv, const GmfCoef& bt) {
+void plain_launch(hipStream_t st, int slot, int kind, int64_t n, const double* A, float* F, float* J, float* H,
+                  double al, const double* alv, const GmfCoef& bt) {
   if (alv) gmf_launch_pow_mixed(st, n, alv, A, F, J, H, bt);
-  else gmf_launch_mixed(st, kind, n, A, F, J, H, al, bt);
+  else gmf_launch_mixed(st, slot, kind, n, A, F, J, H, al, bt);
 }
 
 }  // namespace
@@ -384,7 +384,7 @@ int gmf_sorted_launch(hipStream_t st, int slot, int kind, int order, int64_t n, 
     OT *Fp = F + 9 * lo, *Jp = J ? J + 81 * lo : nullptr, *Hp = H ? H + 729 * lo : nullptr;
     const double* ap = alv ? alv + lo : nullptr;
     if (strat == GMF_SORT_PLAIN) {
-      plain_launch(st, kind, m, Ap, Fp, Jp, Hp, al, ap, bt);
+      plain_launch(st, slot, kind, m, Ap, Fp, Jp, Hp, al, ap, bt);
     } else if (strat == GMF_SORT_STAGED) {
       if (alv) staged_part<POW, OT, true>(st, order, m, Ap, Fp, Jp, Hp, 1.0, ap, bt, work);
       else if (kind == EXP) staged_part<EXP, OT, false>(st, order, m, Ap, Fp, Jp, Hp, al, nullptr, bt, work);

@@ -37,13 +37,13 @@ void launch_chunk(hipStream_t st, int kind, bool mat, int64_t n, const double* x
                   double al, const mf::SmfCoef& c2) {
   mf::smf_launch_mixed(st, kind, mat, n, x, F, J, H, al, c2);
 }
-void glaunch_chunk(hipStream_t st, int kind, int64_t n, const double* A, double* F, double* J, double* H, double al,
-                   const mf::GmfCoef& bt) {
-  mf::gmf_launch(st, kind, n, A, F, J, H, al, bt);
+void glaunch_chunk(hipStream_t st, int slot, int kind, int64_t n, const double* A, double* F, double* J, double* H,
+                   double al, const mf::GmfCoef& bt) {
+  mf::gmf_launch(st, slot, kind, n, A, F, J, H, al, bt);
 }
-void glaunch_chunk(hipStream_t st, int kind, int64_t n, const double* A, float* F, float* J, float* H, double al,
-                   const mf::GmfCoef& bt) {
-  mf::gmf_launch_mixed(st, kind, n, A, F, J, H, al, bt);
+void glaunch_chunk(hipStream_t st, int slot, int kind, int64_t n, const double* A, float* F, float* J, float* H,
+                   double al, const mf::GmfCoef& bt) {
+  mf::gmf_launch_mixed(st, slot, kind, n, A, F, J, H, al, bt);
 }
 
 void pow_chunk(hipStream_t st, bool mat, int64_t n, const double* alv, const double* x, double* F, double* J,
@@ -85,6 +85,7 @@ int smf_call(int kind, double alpha, int64_t n, const double* x, bool mat, OT* F
 template <class OT>
 int gmf_call(int kind, double alpha, int64_t n, const double* A, OT* F, OT* J, OT* H, void* stream) {
   if (!valid(kind, alpha, n, J, H)) return -1;
+  const int slot = kind;
   if (kind == mf::SQRT) kind = mf::POW, alpha = 0.5;
   if (kind != mf::POW) alpha = 0.0;
   if (n == 0) return 0;
@@ -93,7 +94,7 @@ int gmf_call(int kind, double alpha, int64_t n, const double* A, OT* F, OT* J, O
   mf::gmf_coefs(kind, alpha, bt);
   for (int64_t lo = 0; lo < n; lo += mf::MAX_PER_LAUNCH) {
     const int64_t m = n - lo < mf::MAX_PER_LAUNCH ? n - lo : mf::MAX_PER_LAUNCH;
-    glaunch_chunk((hipStream_t)stream, kind, m, A + 9 * lo, F + 9 * lo, J ? J + 81 * lo : nullptr,
+    glaunch_chunk((hipStream_t)stream, slot, kind, m, A + 9 * lo, F + 9 * lo, J ? J + 81 * lo : nullptr,
                   H ? H + 729 * lo : nullptr, alpha, bt);
     if (launched()) return -2;
   }
@@ -289,6 +290,17 @@ extern "C" int gmf_hip_set_sort_strategy(int kind, int order, int mixed, int str
 extern "C" int gmf_hip_get_sort_strategy(int kind, int order, int mixed) {
   if (kind < mf::EXP || kind > 4 || order < 0 || order > 2) return -1;
   return mf::gmf_sort_strategy_get(kind, order, mixed != 0);
+}
+
+extern "C" int gmf_hip_set_tiles(int kind, int order, int mixed, int tiles) {
+  if (kind < mf::EXP || kind > 4 || order < 0 || order > 2 || !(tiles == 1 || tiles == 2 || tiles == 4)) return -1;
+  mf::gmf_tiles_set(kind, order, mixed != 0, tiles);
+  return 0;
+}
+
+extern "C" int gmf_hip_get_tiles(int kind, int order, int mixed) {
+  if (kind < mf::EXP || kind > 4 || order < 0 || order > 2) return -1;
+  return mf::gmf_tiles_get(kind, order, mixed != 0);
 }
 
 extern "C" int smf_hip_set_tile(int kind, int matrix_layout, int mixed, int hpp) {
