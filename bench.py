@@ -508,6 +508,16 @@ def main():
         return
     head = leg("smf fp64", lib.smf_hip_eval_matrix, dA, n, 8, args.steps, args.warmup)
     legs = {"smf_fp64": head}
+    # the write ceiling of the same output shape: mf_hip_store_probe (pure 16-byte stores of F, J, H, no loads, no
+    # arithmetic) over the same buffers, against the headline kernel's write rate (6552 of its 6624 B are stores)
+    probe = lambda: lib.mf_hip_store_probe(n, 9, Fb.data_ptr(), Jb.data_ptr(), Hb.data_ptr(), sh)
+    pms, _ = timed(torch, stream, [probe], max(args.steps, 5), args.warmup)
+    wbytes = n * (9 + 81 + 729) * 8
+    ceiling = {"probe_ms": pms[0], "probe_write_GBs": wbytes / (pms[0] * 1e-3) / 1e9,
+               "kernel_write_GBs": wbytes / (head["avg_kernel_ms"] * 1e-3) / 1e9,
+               "frac_of_write_ceiling": pms[0] / head["avg_kernel_ms"]}
+    log(f"bench: write ceiling (store probe, same shape): {pms[0]:.3f} ms = {ceiling['probe_write_GBs']:.0f} GB/s; "
+        f"headline kernel {ceiling['kernel_write_GBs']:.0f} GB/s = {ceiling['frac_of_write_ceiling']:.1%} of it")
     if not (args.only_headline or args.only_hencky):
         legs["smf_mixed"] = leg("smf mixed", lib.smf_hip_eval_matrix_mixed, dA, n, 4, args.steps, args.warmup,
                                 funcs_mx)
@@ -582,7 +592,8 @@ def main():
                       "layout": "matrix", "bytes_per_matrix": head["bytes_per_matrix"]},
            "roofline": {"bound": "hbm", "achieved": head["achieved_GBs"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
                         "frac": head["frac"], "traffic": profiled_traffic(n), "kernel": "mf::smf_kernel<KIND,true,2,double>",
-                        "avg_kernel_ms": head["avg_kernel_ms"], "measured_copy_GBs": copy_gbs},
+                        "avg_kernel_ms": head["avg_kernel_ms"], "measured_copy_GBs": copy_gbs,
+                        "write_ceiling": ceiling},
            "legs": legs, "tuned_tiles": tiles, "tuned_gmf": gtiles}
     if not (args.only_headline or args.only_hencky):
         out["mixed_vs_fp64"] = legs_c

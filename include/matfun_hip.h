@@ -172,8 +172,10 @@ int mf_hip_batch_eval_mixed(const mf_hip_plan *plan, const double *A, const doub
  *          back to `stream` with events, so the call stays asynchronous and capturable; calls that share a plan
  *          must not overlap)
  *   per_cu > 0: persistent, per_cu workgroups per compute unit loop over the tiles; 0: one workgroup per tile
- * mf_hip_plan_tune times the candidates on the caller's buffers (the batch is evaluated several times) and keeps the
- * fastest; `result` (optional) receives {mode, per_cu, order, microseconds of the best}.  Synchronous. */
+ * mf_hip_plan_tune times the split candidates (modes 1 and 2, one workgroup per tile or 2, 4, 8 per CU; mode 0 is
+ * measured slower on MI355X in every form -- its register use and LDS are those of the largest class for every tile --
+ * and is chosen only through mf_hip_plan_set_strategy) on the caller's buffers (the batch is evaluated several times)
+ * and keeps the fastest; `result` (optional) receives {mode, per_cu, order, microseconds of the best}.  Synchronous. */
 int mf_hip_plan_set_strategy(mf_hip_plan *plan, int mode, int per_cu, int order);
 int mf_hip_plan_tune(mf_hip_plan *plan, const double *A, const double *alpha, void *F, void *J, void *H, int mixed,
                      void *stream, double *result);
@@ -240,6 +242,15 @@ int smf_hip_get_tile(int kind, int matrix_layout, int mixed);
  * timing.  Returns 0 / -1 (set), the choice or -1 (get). */
 int gmf_hip_set_tiles(int kind, int order, int mixed, int tiles);
 int gmf_hip_get_tiles(int kind, int order, int mixed);
+
+/* ---- write-ceiling probe (no counterpart in the reference; a measurement aid): stores a constant pattern into F, J, H
+ * in the layout of an order-2 evaluation of n matrices with nf = 9 (matrix) or 6 (Voigt) values per F row -- per
+ * wavefront 64 matrices' F, J and H blocks, each one contiguous range, with the evaluation kernels' 16-byte stores
+ * and no loads or arithmetic: the rate at which this GPU takes those three output streams.  J / H may be NULL (not
+ * written).  Returns 0, -1 (n < 0, nf not 6 or 9, F NULL with n > 0, H without J, a pointer not 16-byte aligned),
+ * -2 (launch error).  Asynchronous. */
+int mf_hip_store_probe(int64_t n, int nf, double *F, double *J, double *H, void *stream);
+int mf_hip_store_probe_f32(int64_t n, int nf, float *F, float *J, float *H, void *stream);
 
 /* library version ("0.1.3"-compatible API level plus this build); replaces smf_version (c/include/matfun/matfun.h:53) */
 const char *smf_hip_version(void);

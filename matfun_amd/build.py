@@ -14,7 +14,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libmatfun_hip.so")
 SOURCES = ["matfun_hip.hip", "smf_kernels.hip", "gmf_kernels.hip", "gmf_sorted.hip", "batch.hip", "batch_fused.hip",
-           "batch_smf.hip", "batch_gmf.hip", "hencky.hip"]
+           "batch_smf.hip", "batch_gmf.hip", "hencky.hip", "store_probe.hip"]
 # the heterogeneous-batch kernels: MachineLICM off, so that no per-tile constant is hoisted out of the persistent loop
 # over tiles and held in registers across it (batch_kernels.hpp)
 PER_FILE = {"batch_fused.hip": ["-mllvm", "-disable-machine-licm"], "batch_smf.hip": ["-mllvm", "-disable-machine-licm"],

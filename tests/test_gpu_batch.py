@@ -128,7 +128,7 @@ def test_tuner_keeps_results_and_picks_a_strategy():
     dA = torch.from_numpy(A).cuda()
     res = b.tune(dA)
     print(f"tuned: {res}", flush=True)
-    assert res["mode"] in (0, 1) and res["us"] > 0
+    assert res["mode"] in (1, 2) and res["us"] > 0     # (the fused kernel, mode 0, is not a tuning candidate)
     after = run_batch(b, A, None, False)
     for x, y in zip(before, after):
         assert same(x, y)

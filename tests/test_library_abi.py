@@ -63,6 +63,20 @@ def test_invalid_arguments_rejected_before_any_device_work():
 
 
 @pytest.mark.skipif(not os.path.exists(LIB), reason="HIP library not built (python -m matfun_amd.build)")
+def test_store_probe_rejects_before_device_work():
+    lib = ctypes.CDLL(LIB)
+    for f in (lib.mf_hip_store_probe, lib.mf_hip_store_probe_f32):
+        f.argtypes, f.restype = [ctypes.c_int64, ctypes.c_int] + [ctypes.c_void_p] * 4, ctypes.c_int
+        d = ctypes.c_void_p(0x1000)
+        assert f(-1, 9, d, None, None, None) == -1                    # n < 0
+        assert f(10, 7, d, None, None, None) == -1                    # nf not 6 or 9
+        assert f(10, 9, None, None, None, None) == -1                 # F NULL
+        assert f(10, 9, d, None, d, None) == -1                       # H without J
+        assert f(10, 9, ctypes.c_void_p(0x1008), d, d, None) == -1    # not 16-byte aligned
+        assert f(0, 9, None, None, None, None) == 0                   # n == 0: nothing to do
+
+
+@pytest.mark.skipif(not os.path.exists(LIB), reason="HIP library not built (python -m matfun_amd.build)")
 def test_per_matrix_exponent_entry_points_reject_before_device_work():
     lib = ctypes.CDLL(LIB)
     args = [ctypes.c_void_p, ctypes.c_int64] + [ctypes.c_void_p] * 5
