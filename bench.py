@@ -320,6 +320,9 @@ def main():
     ap.add_argument("--batch-elements", type=int, default=1 << 20)
     ap.add_argument("--batch-iters", type=int, default=20)
     ap.add_argument("--no-tune", action="store_true")
+    ap.add_argument("--only-gmf", action="store_true", help="only the general-matrix legs (for the counter runs)")
+    ap.add_argument("--gmf-strategy", type=int, choices=(0, 1, 2), default=None,
+                    help="every function, order and precision of the sorted entry points on this strategy")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this benchmark runs on one GPU only; --gpus must be 1 (the batch shards "
@@ -503,8 +506,26 @@ def main():
         o0["valu"] = profiled_valu()
         return o0
 
+    if args.gmf_strategy is not None:
+        for k in range(5):
+            for o in range(3):
+                for x in range(2):
+                    assert lib.gmf_hip_set_sort_strategy(k, o, x, args.gmf_strategy) == 0
     if args.only_order0:
         print(json.dumps({"order0": order0_legs()}), flush=True)
+        return
+    if args.only_gmf:
+        work = torch.empty(lib.gmf_hip_sort_workspace(ng), dtype=torch.uint8, device=dev)
+        g = {}
+        for o in (2, 1):
+            g[f"fp64_o{o}"] = leg(f"gmf fp64 order {o}", lib.gmf_hip_eval, dG, ng, 8, args.steps, args.warmup, order=o)
+            g[f"mixed_o{o}"] = leg(f"gmf mixed order {o}", lib.gmf_hip_eval_mixed, dG, ng, 4, args.steps, args.warmup,
+                                   order=o)
+            g[f"fp64_o{o}_sorted"] = leg(f"gmf fp64 order {o} sorted", lib.gmf_hip_eval_sorted, dG, ng, 8, args.steps,
+                                         args.warmup, order=o, work=work)
+            g[f"mixed_o{o}_sorted"] = leg(f"gmf mixed order {o} sorted", lib.gmf_hip_eval_sorted_mixed, dG, ng, 4,
+                                          args.steps, args.warmup, order=o, work=work)
+        print(json.dumps({"gmf": g, "strategy": args.gmf_strategy}), flush=True)
         return
     head = leg("smf fp64", lib.smf_hip_eval_matrix, dA, n, 8, args.steps, args.warmup)
     legs = {"smf_fp64": head}
