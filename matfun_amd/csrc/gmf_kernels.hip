@@ -16,7 +16,7 @@ namespace mf {
 // Measured on MI355X over 2^20 matrices of the general mix: sharing takes 10-12% off the FP32 order-1 kernels, which
 // run four workgroups per CU, so that the other workgroups fill the SIMDs while a workgroup's waves wait for its
 // longest series, and 3-4% off order 0; the FP64 order-1 and the order-2 kernels (two or three wavefronts per SIMD by
-// registers) lose 4-13% with two or four
+// registers) lose 4-24% with two or four
 int gmf_nw_default(int slot, int order, bool mixed) {
   const bool longs = slot == LOG || slot == POW || slot == SQRT;
   return longs && (order == 0 || (order == 1 && mixed)) ? 4 : (slot == 4 && order == 1 && mixed ? 4 : 1);
