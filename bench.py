@@ -529,16 +529,25 @@ def main():
         return
     head = leg("smf fp64", lib.smf_hip_eval_matrix, dA, n, 8, args.steps, args.warmup)
     legs = {"smf_fp64": head}
-    # the write ceiling of the same output shape: mf_hip_store_probe (pure 16-byte stores of F, J, H, no loads, no
-    # arithmetic) over the same buffers, against the headline kernel's write rate (6552 of its 6624 B are stores)
+    # the ceilings of the same output shape over the same buffers: mf_hip_store_probe (pure 16-byte stores of F, J,
+    # H, no loads, no arithmetic) against the headline kernel's write rate (6552 of its 6624 B are stores), and
+    # mf_hip_store_probe_in (the same after reading the input rows as the kernel does: its whole traffic, no
+    # arithmetic) against the kernel's time
     probe = lambda: lib.mf_hip_store_probe(n, 9, Fb.data_ptr(), Jb.data_ptr(), Hb.data_ptr(), sh)
-    pms, _ = timed(torch, stream, [probe], max(args.steps, 5), args.warmup)
+    probe_in = lambda: lib.mf_hip_store_probe_in(n, dA.data_ptr(), 9, 9, Fb.data_ptr(), Jb.data_ptr(), Hb.data_ptr(),
+                                                 sh)
+    pms, _ = timed(torch, stream, [probe, probe_in], max(args.steps, 5), args.warmup)
     wbytes = n * (9 + 81 + 729) * 8
     ceiling = {"probe_ms": pms[0], "probe_write_GBs": wbytes / (pms[0] * 1e-3) / 1e9,
                "kernel_write_GBs": wbytes / (head["avg_kernel_ms"] * 1e-3) / 1e9,
-               "frac_of_write_ceiling": pms[0] / head["avg_kernel_ms"]}
+               "frac_of_write_ceiling": pms[0] / head["avg_kernel_ms"],
+               "probe_in_ms": pms[1], "probe_in_GBs": n * 6624 / (pms[1] * 1e-3) / 1e9,
+               "probe_in_frac_hbm": n * 6624 / (pms[1] * 1e-3) / 8e12,
+               "frac_of_io_ceiling": pms[1] / head["avg_kernel_ms"]}
     log(f"bench: write ceiling (store probe, same shape): {pms[0]:.3f} ms = {ceiling['probe_write_GBs']:.0f} GB/s; "
-        f"headline kernel {ceiling['kernel_write_GBs']:.0f} GB/s = {ceiling['frac_of_write_ceiling']:.1%} of it")
+        f"headline kernel {ceiling['kernel_write_GBs']:.0f} GB/s = {ceiling['frac_of_write_ceiling']:.1%} of it; "
+        f"with the input reads (the kernel's whole traffic): {pms[1]:.3f} ms = {ceiling['probe_in_GBs']:.0f} GB/s, "
+        f"kernel at {ceiling['frac_of_io_ceiling']:.1%} of it")
     if not (args.only_headline or args.only_hencky):
         legs["smf_mixed"] = leg("smf mixed", lib.smf_hip_eval_matrix_mixed, dA, n, 4, args.steps, args.warmup,
                                 funcs_mx)

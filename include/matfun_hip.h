@@ -251,6 +251,12 @@ int gmf_hip_get_tiles(int kind, int order, int mixed);
  * -2 (launch error).  Asynchronous. */
 int mf_hip_store_probe(int64_t n, int nf, double *F, double *J, double *H, void *stream);
 int mf_hip_store_probe_f32(int64_t n, int nf, float *F, float *J, float *H, void *stream);
+/* the same after reading the input rows x (n x nin doubles, nin = 9 or 6; coalesced through LDS as the evaluation
+ * kernels read them), storing values that depend on them: the whole traffic of an evaluation (read x, write F, J, H)
+ * with no arithmetic.  -1 also for x NULL with n > 0 or nin not 6 or 9. */
+int mf_hip_store_probe_in(int64_t n, const double *x, int nin, int nf, double *F, double *J, double *H, void *stream);
+int mf_hip_store_probe_in_f32(int64_t n, const double *x, int nin, int nf, float *F, float *J, float *H,
+                              void *stream);
 
 /* library version ("0.1.3"-compatible API level plus this build); replaces smf_version (c/include/matfun/matfun.h:53) */
 const char *smf_hip_version(void);

@@ -65,6 +65,9 @@ def library():
         lib.gmf_hip_set_tiles.argtypes = [ctypes.c_int] * 4
         for fn in (lib.mf_hip_store_probe, lib.mf_hip_store_probe_f32):
             fn.argtypes, fn.restype = [ctypes.c_int64, ctypes.c_int] + [ctypes.c_void_p] * 4, ctypes.c_int
+        for fn in (lib.mf_hip_store_probe_in, lib.mf_hip_store_probe_in_f32):
+            fn.argtypes = [ctypes.c_int64, ctypes.c_void_p, ctypes.c_int, ctypes.c_int] + [ctypes.c_void_p] * 4
+            fn.restype = ctypes.c_int
         lib.gmf_hip_get_tiles.argtypes = [ctypes.c_int] * 3
         for fn in (lib.mf_hip_hencky, lib.mf_hip_hencky_mixed):
             fn.argtypes, fn.restype = [ctypes.c_int64] + [ctypes.c_void_p] * 7, ctypes.c_int

@@ -74,6 +74,16 @@ def test_store_probe_rejects_before_device_work():
         assert f(10, 9, d, None, d, None) == -1                       # H without J
         assert f(10, 9, ctypes.c_void_p(0x1008), d, d, None) == -1    # not 16-byte aligned
         assert f(0, 9, None, None, None, None) == 0                   # n == 0: nothing to do
+    for f in (lib.mf_hip_store_probe_in, lib.mf_hip_store_probe_in_f32):
+        f.argtypes = [ctypes.c_int64, ctypes.c_void_p, ctypes.c_int, ctypes.c_int] + [ctypes.c_void_p] * 4
+        f.restype = ctypes.c_int
+        d = ctypes.c_void_p(0x1000)
+        assert f(10, None, 9, 9, d, None, None, None) == -1           # no input rows
+        assert f(10, d, 7, 9, d, None, None, None) == -1              # nin not 6 or 9
+        assert f(10, d, 9, 7, d, None, None, None) == -1              # nf not 6 or 9
+        assert f(-1, d, 9, 9, d, None, None, None) == -1              # n < 0
+        assert f(10, d, 9, 9, d, None, d, None) == -1                 # H without J
+        assert f(0, None, 9, 9, None, None, None, None) == 0          # n == 0: nothing to do
 
 
 @pytest.mark.skipif(not os.path.exists(LIB), reason="HIP library not built (python -m matfun_amd.build)")
