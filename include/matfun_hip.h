@@ -77,9 +77,10 @@ int gmf_hip_eval_pow_mixed(const double *alpha, int64_t n, const double *A, floa
  * process-wide strategy (gmf_hip_set_sort_strategy; defaults measured on MI355X, matfun_amd.tune times them):
  *   0  no sorting: the kernel of gmf_hip_eval (bitwise its results)
  *   1  the matrices in length order through the same kernel, each writing its own rows (bitwise its results)
- *   2  staged: per matrix its nodes and series inputs (input order), the series in length order, the rest in input
- *      order with contiguous stores -- gmf_hip_eval's results up to rounding (per matrix normwise <= 1e-13; measured
- *      at most 5e-14 on the general mix, with the same error distribution against the reference)
+ *   2  staged: per matrix its nodes and series inputs (input order), the series in the order of class and length
+ *      (real arithmetic for the wavefronts whose group offsets are all real), the rest in input order with contiguous
+ *      stores -- gmf_hip_eval's results up to rounding (per matrix normwise <= 1e-13 plus a few times gmf_hip_eval's
+ *      own distance to the reference on that matrix; the same error distribution against the reference)
  * `work` is caller-owned device memory of work_bytes bytes; gmf_hip_sort_workspace(n) bytes (about 540 bytes per
  * matrix plus 200 KB) take any batch of n in one pass; a smaller workspace runs the batch in consecutive parts (a
  * staged strategy uses the permuted one when the workspace holds fewer than 2^15 matrices of it).  Same arguments,

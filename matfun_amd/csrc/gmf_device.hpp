@@ -269,6 +269,66 @@ MF_DEV void gmf_series(const GmfSeriesIn& s, const double* __restrict__ bt, cpx*
   }
 }
 
+// the class of a lane's group series: 0 real offsets and no pole node (a group of all three nodes, or none), 1 real
+// offsets with a pole node (a pair group of real nodes), 2 complex offsets (a conjugate pair, or a triple with one)
+MF_DEV int gmf_series_class(const GmfSeriesIn& s) {
+  const bool real = s.u[0].im == 0.0 && s.u[1].im == 0.0 && s.u[2].im == 0.0;
+  return !real ? 2 : ((s.flags & 56) != 0 ? 1 : 0);
+}
+
+// gmf_series for a lane whose group offsets u are real: the dual vectors E and the sums stay real (their imaginary
+// parts are zero in gmf_series), so the recurrence runs in real arithmetic -- two FP64 operations per entry and term
+// instead of six.  POLE = false: no lane of the caller's wavefront has a pole node, so every entry takes b_j (none of
+// the factor selects; the lagged terms past K are not needed either).  The sums equal gmf_series' up to rounding (the
+// two loops contract into FMAs differently); a lane whose offsets are not real must not call it.
+template <int KIND, int ORDER, bool PA, bool POLE>
+MF_DEV void gmf_series_real(const GmfSeriesIn& s, const double* __restrict__ bt, cpx* acc) {
+  constexpr int ND = ORDER == 0 ? 3 : (ORDER == 1 ? 9 : 19), LAG = ORDER + 1;
+  const bool pk[3] = {(s.flags & 8) != 0, (s.flags & 16) != 0, (s.flags & 32) != 0};
+  const bool nopole = !(pk[0] || pk[1] || pk[2]);
+  const double iap = s.iap, al = s.al;
+  const double u[3] = {s.u[0].re, s.u[1].re, s.u[2].re};
+  double E[ND], a[ND];
+#pragma unroll
+  for (int c = 0; c < ND; c++) E[c] = a[c] = 0.0;
+  double q1 = 0.0, q2 = 0.0, q3 = 0.0, r2 = 0.0, r3a = 0.0, r3b = 0.0, bnext = 0.0;
+  const int KT = s.K > 0 ? s.K + (POLE ? LAG : 0) : 0;
+  for (int j = 0; j < KT; j++) {
+    const double bl = j == 0 ? s.b0 : bnext;
+    bnext = PA ? bl * ((al - j) * bt[j + 1 < GMF_KMAX ? j + 1 : GMF_KMAX - 1])
+               : bt[j + 1 < GMF_KMAX ? j + 1 : GMF_KMAX - 1];
+    const double bj = POLE ? (j < s.K ? bl : 0.0) : bl;
+    double f1 = bj, f2 = bj, f3 = bj;
+    if constexpr (POLE) {
+      f1 = nopole ? bj : (j - 1 < s.K ? q1 : 0.0);
+      f2 = nopole ? bj : (j - 2 < s.K ? r2 : 0.0);
+      f3 = nopole ? bj : (j - 3 < s.K ? r3b : 0.0);
+      r3b = r3a; r3a = q3; r2 = q2;
+      const double g1 = (bl - q1) * iap, g2 = (g1 - q2) * iap, g3 = (g2 - q3) * iap;
+      q1 = g1; q2 = g2; q3 = g3;
+    }
+#pragma unroll
+    for (int c = ND - 1; c >= 0; c--) {
+      const int par = dd_parent(c), i = dd_added(c);
+      const double epp = par < 0 ? (j == 0 ? 1.0 : 0.0) : E[par];
+      E[c] = epp + u[i] * E[c];
+    }
+#pragma unroll
+    for (int c = 0; c < ND; c++) {
+      double g = bj;
+      if constexpr (POLE) {
+        const double qa = dd_a(c) == 0 ? bj : (dd_a(c) == 1 ? f1 : (dd_a(c) == 2 ? f2 : f3));
+        const double qb = dd_b(c) == 0 ? bj : (dd_b(c) == 1 ? f1 : (dd_b(c) == 2 ? f2 : f3));
+        const double qc = dd_c(c) == 0 ? bj : (dd_c(c) == 1 ? f1 : (dd_c(c) == 2 ? f2 : f3));
+        g = pk[0] ? qa : (pk[1] ? qb : qc);
+      }
+      a[c] += E[c] * g;
+    }
+  }
+#pragma unroll
+  for (int c = 0; c < ND; c++) acc[c] = mk(a[c], 0.0);
+}
+
 // the series computed by the lane itself
 template <int KIND, int ORDER, bool PA>
 struct GmfSeriesLocal {
@@ -300,6 +360,7 @@ struct GmfOut {
   double D3[27];      // ... on three indices (order >= 2)
   double sc1, sc2;    // factors of J and H (NaN for a matrix outside the domain)
   int K;              // series length (set only by a GmfSeriesLength pass)
+  int cls;            // gmf_series_class of the series (set only by a GmfSeriesLength pass)
 };
 
 // what the rest of a matrix's evaluation needs from its nodes, once the group series is set up (gmf_nodes ->
@@ -749,6 +810,7 @@ MF_DEV void gmf_lane(const double* a, double al, const double* __restrict__ bt, 
   gmf_nodes<KIND, ORDER, PA>(a, al, bt, rinv, st, sin);
   if constexpr (SER::K_ONLY) {
     o.K = sin.K;
+    o.cls = gmf_series_class(sin);
     series.post(sin, st);
     return;
   }

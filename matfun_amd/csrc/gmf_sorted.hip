@@ -66,11 +66,13 @@ __global__ void __launch_bounds__(KLEN_THREADS)
   }
   GmfOut o;
   gmf_lane<KIND, ORDER, PA>(a, al, bt.b, o, g_gmf_recip.r, GmfSeriesLength{sin + i, nodes + i});
-  key[i] = (uint8_t)(GMF_KMAX - o.K);
+  key[i] = (uint8_t)(o.cls == 0 ? GMF_KMAX - o.K : GMF_KMAX + 1 + (o.cls - 1) * KQ + (GMF_KMAX - o.K) / 8);
 }
 
 // the group series of the matrices in sorted order: lane t of the batch takes matrix perm[t] (its inputs from
-// sin, its ND sums to sums at the matrix's own row)
+// sin, its ND sums to sums at the matrix's own row).  The keys order the series by class, so most wavefronts hold
+// one class: a wavefront whose offsets are all real runs the real loop (without the pole factors when no lane has a
+// pole node), any other the complex loop (the choice is the wavefront's, by ballot: no divergence)
 template <int KIND, int ORDER, bool PA>
 __global__ void __launch_bounds__(KLEN_THREADS)
     gmf_series_kernel(int64_t n, const int32_t* __restrict__ perm, const GmfSeriesIn* __restrict__ sin, GmfCoef bt,
@@ -79,8 +81,12 @@ __global__ void __launch_bounds__(KLEN_THREADS)
   const int64_t t = (int64_t)blockIdx.x * KLEN_THREADS + threadIdx.x;
   if (t >= n) return;
   const int64_t i = perm[t];
+  const GmfSeriesIn s = sin[i];
+  const int cls = gmf_series_class(s);
   cpx acc[ND];
-  gmf_series<KIND, ORDER, PA>(sin[i], bt.b, acc);
+  if (__ballot(cls != 0) == 0) gmf_series_real<KIND, ORDER, PA, false>(s, bt.b, acc);
+  else if (__ballot(cls == 2) == 0) gmf_series_real<KIND, ORDER, PA, true>(s, bt.b, acc);
+  else gmf_series<KIND, ORDER, PA>(s, bt.b, acc);
 #pragma unroll
   for (int c = 0; c < ND; c++) sums[i * ND + c] = acc[c];
 }
@@ -314,13 +320,13 @@ std::atomic<int> g_strategy[5][3][2];
 std::once_flag g_strategy_once;
 
 // defaults measured on MI355X (general mix, 2^20 matrices): short series (exp) and order 0 gain nothing from the
-// length order (its passes cost more than the shorter loops save); at order 1 the staged evaluation is 20% faster
-// for log / powers, at order 2 for the mixed form 14%, while the FP64 order-2 kernel, bound by its stores with the
-// series hidden behind them, is as fast unsorted
+// length order (its passes cost more than the shorter loops save); for log / powers the staged evaluation is faster
+// at order 1 (20-29%) and at order 2 (mixed 24-25%, FP64 3-10%: its series in real arithmetic for the wavefronts
+// whose offsets are all real)
 int default_strategy(int slot, int order, int mixed) {
+  (void)mixed;
   if (slot == 0 || order == 0) return GMF_SORT_PLAIN;
-  if (order == 1) return GMF_SORT_STAGED;
-  return mixed ? GMF_SORT_STAGED : GMF_SORT_PLAIN;
+  return GMF_SORT_STAGED;
 }
 
 void init_strategies() {

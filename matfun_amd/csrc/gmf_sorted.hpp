@@ -8,7 +8,12 @@ namespace mf {
 
 struct GmfCoef;
 
-constexpr int KBINS = 201;           // keys 200 - K for series lengths K = 0 .. 200
+// sort keys: the permuted evaluation 200 - K (series lengths K = 0 .. 200, longest first); the staged evaluation
+// orders by series class first (gmf_series_class: real offsets without a pole, 200 - K; real with a pole and complex,
+// 201 + (200 - K) / 8 and 227 + (200 - K) / 8), so that a wavefront of its series pass holds one class and runs the
+// class's own loop
+constexpr int KQ = (200 / 8) + 1;
+constexpr int KBINS = 201 + 2 * KQ;
 constexpr int KSORT_MAXCH = 128;     // histogram chunks of one part at most
 
 // evaluation strategies of the workspace entry points (gmf_hip_eval_sorted*), per function slot (0 exp, 1 log, 2 pow,

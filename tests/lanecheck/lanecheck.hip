@@ -95,7 +95,62 @@ void gmf_all(double al, const double* alv, int64_t n, const double* A, double* F
   }
 }
 
+// the group series of each matrix through the complex loop (gmf_series) and, for a lane whose offsets are real,
+// through the real loops (gmf_series_real with and, without a pole node, without the pole factors): cls[m] the
+// lane's class, err[3 m ..] the normwise relative differences of the real loops' sums (0 where not run) and the
+// largest imaginary part of the complex loop's sums for a real lane
+template <int KIND, int ORDER>
+void gmf_series_check(double al, int64_t n, const double* A, int* cls, double* err) {
+  constexpr int ND = ORDER == 0 ? 3 : (ORDER == 1 ? 9 : 19);
+  mf::GmfCoef bt;
+  mf::gmf_coefs(KIND, al, bt);
+  static constexpr mf::GmfRecip rinv = mf::gmf_recip_table();
+  for (int64_t m = 0; m < n; m++) {
+    mf::GmfNodes st;
+    mf::GmfSeriesIn s;
+    mf::gmf_nodes<KIND, ORDER, false>(A + 9 * m, al, bt.b, rinv.r, st, s);
+    mf::cpx ref[ND], r0[ND], r1[ND];
+    mf::gmf_series<KIND, ORDER, false>(s, bt.b, ref);
+    const int c = mf::gmf_series_class(s);
+    cls[m] = c;
+    double nrm = 0.0, d0 = 0.0, d1 = 0.0, imax = 0.0;
+    for (int k = 0; k < ND; k++) nrm += ref[k].re * ref[k].re + ref[k].im * ref[k].im;
+    if (c <= 1) {
+      mf::gmf_series_real<KIND, ORDER, false, true>(s, bt.b, r1);
+      for (int k = 0; k < ND; k++) {
+        d1 += (r1[k].re - ref[k].re) * (r1[k].re - ref[k].re) + (r1[k].im - ref[k].im) * (r1[k].im - ref[k].im);
+        imax = fabs(ref[k].im) > imax ? fabs(ref[k].im) : imax;
+      }
+    }
+    if (c == 0) {
+      mf::gmf_series_real<KIND, ORDER, false, false>(s, bt.b, r0);
+      for (int k = 0; k < ND; k++)
+        d0 += (r0[k].re - ref[k].re) * (r0[k].re - ref[k].re) + (r0[k].im - ref[k].im) * (r0[k].im - ref[k].im);
+    }
+    err[3 * m] = nrm > 0 ? sqrt(d0 / nrm) : sqrt(d0);
+    err[3 * m + 1] = nrm > 0 ? sqrt(d1 / nrm) : sqrt(d1);
+    err[3 * m + 2] = imax;
+  }
+}
+
 }  // namespace
+
+extern "C" int lanecheck_gmf_series_real(int kind, double al, int order, int64_t n, const double* A, int* cls,
+                                         double* err) {
+  if (kind == mf::SQRT) kind = mf::POW, al = 0.5;
+  if (kind != mf::POW) al = 0.0;
+#define MF_SC(K)                                                         \
+  if (order == 2) gmf_series_check<K, 2>(al, n, A, cls, err);            \
+  else if (order == 1) gmf_series_check<K, 1>(al, n, A, cls, err);       \
+  else gmf_series_check<K, 0>(al, n, A, cls, err);
+  switch (kind) {
+    case mf::EXP: MF_SC(mf::EXP) break;
+    case mf::LOG: MF_SC(mf::LOG) break;
+    default: MF_SC(mf::POW) break;
+  }
+#undef MF_SC
+  return 0;
+}
 
 static int smf_entry(bool mat, int kind, double al, int64_t n, const double* x, double* F, double* J, double* H) {
   if (kind != mf::POW) al = kind == mf::SQRT ? 0.5 : 0.0;

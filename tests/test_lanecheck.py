@@ -217,3 +217,24 @@ def test_fused_hencky_lane_math_vs_oracle(lane):
         assert e.max() <= 1e-12, (name, e.max(), int(np.argmax(e)))
     assert np.isnan(K[11]).all() and not np.isnan(np.delete(K, 11, axis=0)).any()
 
+
+
+@pytest.mark.parametrize("kind,alpha", [("exp", 0.0), ("log", 0.0), ("sqrt", 0.0), ("pow", 0.3), ("pow", 60.5)])
+def test_gmf_real_series_loops_equal_the_complex_loop(lane, kind, alpha):
+    """the real-arithmetic group series (gmf_series_real, run by the staged evaluation for wavefronts whose offsets
+    are all real) against the complex loop on the general mix: sums within 1e-14 (normwise), and the complex loop's
+    imaginary parts exactly zero for those lanes, so the real loop drops nothing"""
+    lane.lanecheck_gmf_series_real.argtypes = [ctypes.c_int, ctypes.c_double, ctypes.c_int, ctypes.c_int64,
+                                               ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]
+    A = np.ascontiguousarray(W.general(20000, seed=41))
+    for order in (0, 1, 2):
+        cls = np.empty(len(A), dtype=np.int32)
+        err = np.empty((len(A), 3))
+        lane.lanecheck_gmf_series_real(KC[kind], alpha, order, len(A), A.ctypes.data, cls.ctypes.data,
+                                       err.ctypes.data)
+        counts = np.bincount(cls, minlength=3)
+        assert counts[0] > 1000 and counts[1] > 100 and counts[2] > 100, counts     # (every class occurs)
+        real = cls <= 1
+        assert err[real, 2].max() == 0.0, (kind, order)
+        assert err[cls == 0, 0].max() <= 1e-14, (kind, order, err[cls == 0, 0].max())
+        assert err[real, 1].max() <= 1e-14, (kind, order, err[real, 1].max())
