@@ -338,16 +338,20 @@ struct GmfSeriesLocal {
 };
 
 // no series: gmf_lane stops once the series inputs are known, returns the length in GmfOut::K and hands the inputs
-// and the nodes to post() (the first pass of the length-sorted evaluation, gmf_sorted.hip; NULL: not stored)
-struct GmfNodes;
+// and the nodes to post() (the first pass of the length-sorted evaluation, gmf_sorted.hip; NULL: not stored): the
+// nodes staged as gmf_stage_nodes writes them
+struct GmfNodesRe;
+template <class N>
+MF_DEV void gmf_stage_nodes(const N& st, int cls, GmfNodesRe& r, double* im);
 struct GmfSeriesLength {
   static constexpr bool K_ONLY = true;
   GmfSeriesIn* out = nullptr;
-  GmfNodes* nodes = nullptr;
+  GmfNodesRe* nre = nullptr;
+  double* nim = nullptr;
   template <class N>
   MF_DEV void post(const GmfSeriesIn& s, const N& st) const {
     if (out) *out = s;
-    if (nodes) *nodes = st;
+    if (nre) gmf_stage_nodes(st, gmf_series_class(s), *nre, nim);
   }
   MF_DEV void operator()(const GmfSeriesIn&, cpx*) const {}
 };
@@ -372,6 +376,68 @@ struct GmfNodes {
   double m, sig, xshift, cxg, L;
   unsigned flags, pad;
 };
+
+// the nodes as the staged evaluation keeps them between its passes, in two parts so that the passes that read them
+// move fewer bytes (reads amid the finishing pass's stores cost it several times their share of the traffic): the
+// real record (96 B) of every matrix, and the imaginary parts of x and mu (6 doubles) only for a matrix that has one
+// other than +0 (flags bit 8); flags bits 0..7 as GmfNodes, bits 9..10 the series class (gmf_series_class: sums with
+// imaginary parts only for class 2)
+struct GmfNodesRe {
+  double xr[3], mur[3];
+  double m, sig, xshift, cxg, L;
+  unsigned flags, pad;
+};
+constexpr unsigned GMF_NODES_CPLX = 256u;
+template <class N>
+MF_DEV void gmf_stage_nodes(const N& st, int cls, GmfNodesRe& r, double* im) {
+  bool cn = false;
+#pragma unroll
+  for (int i = 0; i < 3; i++) {
+    cn = cn || st.x[i].im != 0.0 || __builtin_signbit(st.x[i].im) || st.mu[i].im != 0.0 ||
+         __builtin_signbit(st.mu[i].im);
+    r.xr[i] = st.x[i].re;
+    r.mur[i] = st.mu[i].re;
+  }
+  r.m = st.m;
+  r.sig = st.sig;
+  r.xshift = st.xshift;
+  r.cxg = st.cxg;
+  r.L = st.L;
+  r.flags = st.flags | (cn ? GMF_NODES_CPLX : 0u) | ((unsigned)cls << 9);
+  r.pad = 0;
+  if (cn) {
+#pragma unroll
+    for (int i = 0; i < 3; i++) {
+      im[i] = st.x[i].im;
+      im[3 + i] = st.mu[i].im;
+    }
+  }
+}
+// the nodes back from the two parts (imaginary parts +0 unless stored)
+MF_DEV GmfNodes gmf_unstage_nodes(const GmfNodesRe& r, const double* im) {
+  GmfNodes st;
+  double xi[3] = {0.0, 0.0, 0.0}, mi[3] = {0.0, 0.0, 0.0};
+  if (r.flags & GMF_NODES_CPLX) {
+#pragma unroll
+    for (int i = 0; i < 3; i++) {
+      xi[i] = im[i];
+      mi[i] = im[3 + i];
+    }
+  }
+#pragma unroll
+  for (int i = 0; i < 3; i++) {
+    st.x[i] = mk(r.xr[i], xi[i]);
+    st.mu[i] = mk(r.mur[i], mi[i]);
+  }
+  st.m = r.m;
+  st.sig = r.sig;
+  st.xshift = r.xshift;
+  st.cxg = r.cxg;
+  st.L = r.L;
+  st.flags = r.flags & 255u;
+  st.pad = 0;
+  return st;
+}
 
 // PA (powers with a per-matrix exponent al): bt holds 1 / j instead of the coefficients b_j (gmf_recips)
 // rinv: gmf_recip_table().  The nodes and the group series' inputs of one matrix (gmf_kernel.hpp:251-476)

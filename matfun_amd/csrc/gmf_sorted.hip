@@ -54,7 +54,8 @@ __global__ void __launch_bounds__(KLEN_THREADS)
 template <int KIND, int ORDER, bool PA>
 __global__ void __launch_bounds__(KLEN_THREADS)
     gmf_prep_kernel(int64_t n, const double* __restrict__ A, double al, const double* __restrict__ alv, GmfCoef bt,
-                    uint8_t* __restrict__ key, GmfSeriesIn* __restrict__ sin, GmfNodes* __restrict__ nodes) {
+                    uint8_t* __restrict__ key, GmfSeriesIn* __restrict__ sin, GmfNodesRe* __restrict__ nre,
+                    double* __restrict__ nim) {
   const int64_t i = (int64_t)blockIdx.x * KLEN_THREADS + threadIdx.x;
   if (i >= n) return;
   double a[9];
@@ -65,18 +66,19 @@ __global__ void __launch_bounds__(KLEN_THREADS)
     al = fabs(al) <= 1.7976931348623157e308 ? al : 1.0;
   }
   GmfOut o;
-  gmf_lane<KIND, ORDER, PA>(a, al, bt.b, o, g_gmf_recip.r, GmfSeriesLength{sin + i, nodes + i});
+  gmf_lane<KIND, ORDER, PA>(a, al, bt.b, o, g_gmf_recip.r, GmfSeriesLength{sin + i, nre + i, nim + 6 * i});
   key[i] = (uint8_t)(o.cls == 0 ? GMF_KMAX - o.K : GMF_KMAX + 1 + (o.cls - 1) * KQ + (GMF_KMAX - o.K) / 8);
 }
 
 // the group series of the matrices in sorted order: lane t of the batch takes matrix perm[t] (its inputs from
-// sin, its ND sums to sums at the matrix's own row).  The keys order the series by class, so most wavefronts hold
-// one class: a wavefront whose offsets are all real runs the real loop (without the pole factors when no lane has a
-// pole node), any other the complex loop (the choice is the wavefront's, by ballot: no divergence)
+// sin, its ND sums at the matrix's own row: real parts to sre, imaginary parts to sim for series class 2 only).  The
+// keys order the series by class, so most wavefronts hold one class: a wavefront whose offsets are all real runs the
+// real loop (without the pole factors when no lane has a pole node), any other the complex loop (the choice is the
+// wavefront's, by ballot: no divergence)
 template <int KIND, int ORDER, bool PA>
 __global__ void __launch_bounds__(KLEN_THREADS)
     gmf_series_kernel(int64_t n, const int32_t* __restrict__ perm, const GmfSeriesIn* __restrict__ sin, GmfCoef bt,
-                      cpx* __restrict__ sums) {
+                      double* __restrict__ sre, double* __restrict__ sim) {
   constexpr int ND = ORDER == 0 ? 3 : (ORDER == 1 ? 9 : 19);
   const int64_t t = (int64_t)blockIdx.x * KLEN_THREADS + threadIdx.x;
   if (t >= n) return;
@@ -88,7 +90,11 @@ __global__ void __launch_bounds__(KLEN_THREADS)
   else if (__ballot(cls == 2) == 0) gmf_series_real<KIND, ORDER, PA, true>(s, bt.b, acc);
   else gmf_series<KIND, ORDER, PA>(s, bt.b, acc);
 #pragma unroll
-  for (int c = 0; c < ND; c++) sums[i * ND + c] = acc[c];
+  for (int c = 0; c < ND; c++) sre[i * ND + c] = acc[c].re;
+  if (cls == 2) {
+#pragma unroll
+    for (int c = 0; c < ND; c++) sim[i * ND + c] = acc[c].im;
+  }
 }
 
 // the tile body of the homogeneous kernel over consecutive matrices, each finished from its staged nodes and sums
@@ -263,8 +269,8 @@ void sorted_part(hipStream_t st, int order, int64_t m, const double* A, OT* F, O
 size_t staged_bytes(int64_t m, int order) {
   const int nd = order == 0 ? 3 : (order == 1 ? 9 : 19);
   return align256((size_t)m) + align256(4 * (size_t)m) + 2 * align256(4 * (size_t)KSORT_MAXCH * KBINS) +
-         align256(sizeof(GmfSeriesIn) * (size_t)m) + align256(sizeof(GmfNodes) * (size_t)m) +
-         align256(sizeof(cpx) * nd * (size_t)m);
+         align256(sizeof(GmfSeriesIn) * (size_t)m) + align256(sizeof(GmfNodesRe) * (size_t)m) +
+         align256(6 * sizeof(double) * (size_t)m) + 2 * align256(sizeof(double) * nd * (size_t)m);
 }
 
 template <int KIND, int ORDER, class OT, bool PA>
@@ -274,22 +280,27 @@ void staged_order(hipStream_t st, int64_t m, const double* A, OT* F, OT* J, OT* 
   unsigned char* p = reinterpret_cast<unsigned char*>(s.off) + align256(4 * (size_t)KSORT_MAXCH * KBINS);
   GmfSeriesIn* sin = reinterpret_cast<GmfSeriesIn*>(p);
   p += align256(sizeof(GmfSeriesIn) * (size_t)m);
-  GmfNodes* nodes = reinterpret_cast<GmfNodes*>(p);
-  p += align256(sizeof(GmfNodes) * (size_t)m);
-  cpx* sums = reinterpret_cast<cpx*>(p);
+  constexpr int ND = ORDER == 0 ? 3 : (ORDER == 1 ? 9 : 19);
+  GmfNodesRe* nre = reinterpret_cast<GmfNodesRe*>(p);
+  p += align256(sizeof(GmfNodesRe) * (size_t)m);
+  double* nim = reinterpret_cast<double*>(p);
+  p += align256(6 * sizeof(double) * (size_t)m);
+  double* sre = reinterpret_cast<double*>(p);
+  p += align256(sizeof(double) * ND * (size_t)m);
+  double* sim = reinterpret_cast<double*>(p);
   int64_t chunk = (m + KSORT_MAXCH - 1) / KSORT_MAXCH;
   chunk = (chunk + SORT_THREADS - 1) / SORT_THREADS * SORT_THREADS;
   const int nch = (int)((m + chunk - 1) / chunk);
   const unsigned gl = (unsigned)((m + KLEN_THREADS - 1) / KLEN_THREADS);
   hipLaunchKernelGGL((gmf_prep_kernel<KIND, ORDER, PA>), dim3(gl), dim3(KLEN_THREADS), 0, st, m, A, al, alv, bt, s.key,
-                     sin, nodes);
+                     sin, nre, nim);
   hipLaunchKernelGGL(key_hist_kernel, dim3(nch), dim3(SORT_THREADS), 0, st, m, chunk, s.key, s.hist);
   hipLaunchKernelGGL(key_scan_kernel, dim3(1), dim3(SORT_THREADS), 0, st, nch, s.hist, s.off);
   hipLaunchKernelGGL(key_scatter_kernel, dim3(nch), dim3(SORT_THREADS), 0, st, m, chunk, s.key, s.off, s.perm);
   hipLaunchKernelGGL((gmf_series_kernel<KIND, ORDER, PA>), dim3(gl), dim3(KLEN_THREADS), 0, st, m, s.perm, sin, bt,
-                     sums);
+                     sre, sim);
   hipLaunchKernelGGL((gmf_finish_kernel<KIND, ORDER, OT, PA>), dim3((unsigned)((m + WAVE - 1) / WAVE)), dim3(WAVE), 0,
-                     st, m, A, F, J, H, al, alv, bt, GmfStaged{nodes, sums});
+                     st, m, A, F, J, H, al, alv, bt, GmfStaged{nre, nim, sre, sim});
 }
 
 template <int KIND, class OT, bool PA>
@@ -376,8 +387,8 @@ int gmf_sorted_launch(hipStream_t st, int slot, int kind, int order, int64_t n, 
   int64_t cap = 0;
   if (strat == GMF_SORT_STAGED) {
     const int nd = order == 0 ? 3 : (order == 1 ? 9 : 19);
-    cap = capacity_of(bytes, 5 + sizeof(GmfSeriesIn) + sizeof(GmfNodes) + nd * sizeof(cpx),
-                      [order](int64_t m) { return staged_bytes(m, order); });
+    const size_t per = 5 + sizeof(GmfSeriesIn) + sizeof(GmfNodesRe) + (6 + 2 * nd) * sizeof(double);
+    cap = capacity_of(bytes, per, [order](int64_t m) { return staged_bytes(m, order); });
     if (cap < (n < STAGED_MIN_PART ? n : STAGED_MIN_PART)) strat = GMF_SORT_PERM;   // (too little room: permuted)
   }
   if (strat == GMF_SORT_PERM) cap = gmf_sort_capacity(bytes);

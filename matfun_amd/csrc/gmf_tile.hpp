@@ -103,10 +103,13 @@ struct GmfLds {
 // share out the series (GmfSeriesShared; pool: its LDS, tid = threadIdx.x).  Ends with the tile's outputs written and
 // a wave barrier (the LDS may be reused).
 // the nodes and group series sums of each matrix computed by earlier passes (the staged evaluation, gmf_sorted.hip):
-// nodes[row] and sums[row ND .. row ND + ND - 1]
+// nodes nre[row] and nim[6 row ..] (gmf_stage_nodes), the real parts of the sums sre[row ND ..], their imaginary
+// parts sim[row ND ..] for a matrix of series class 2 (else +0)
 struct GmfStaged {
-  const GmfNodes* nodes;
-  const cpx* sums;
+  const GmfNodesRe* nre;
+  const double* nim;
+  const double* sre;
+  const double* sim;
 };
 
 // STAGED: each lane finishes its matrix from staged nodes and sums (gmf_finish) instead of running gmf_lane
@@ -114,7 +117,7 @@ template <int KIND, int ORDER, class OT, bool PA, class R, int NW = 1, bool STAG
 MF_DEV void gmf_tile(const R& rows, int nm, const double* __restrict__ A, OT* __restrict__ F, OT* __restrict__ J,
                      OT* __restrict__ H, double al, const double* __restrict__ alv, const double* __restrict__ bt,
                      unsigned char* lds, int lane, unsigned char* pool = nullptr, int tid = 0,
-                     GmfStaged staged = GmfStaged{nullptr, nullptr}) {
+                     GmfStaged staged = GmfStaged{nullptr, nullptr, nullptr, nullptr}) {
   using L = GmfLds<ORDER, OT>;
   constexpr int VEC = 16 / (int)sizeof(OT);             // (elements per 16-byte store)
   double* xin = reinterpret_cast<double*>(lds);
@@ -139,10 +142,15 @@ MF_DEV void gmf_tile(const R& rows, int nm, const double* __restrict__ A, OT* __
   if constexpr (STAGED) {
     constexpr int ND = ORDER == 0 ? 3 : (ORDER == 1 ? 9 : 19);
     const int64_t row = rows.m(lane < nm ? lane : 0);
-    const GmfNodes st = staged.nodes[row];
+    const GmfNodesRe nr = staged.nre[row];
+    const GmfNodes st = gmf_unstage_nodes(nr, staged.nim + 6 * row);
     cpx acc[ND];
 #pragma unroll
-    for (int c = 0; c < ND; c++) acc[c] = staged.sums[row * ND + c];
+    for (int c = 0; c < ND; c++) acc[c] = mk(staged.sre[row * ND + c], 0.0);
+    if (((nr.flags >> 9) & 3u) == 2u) {
+#pragma unroll
+      for (int c = 0; c < ND; c++) acc[c].im = staged.sim[row * ND + c];
+    }
     gmf_finish<KIND, ORDER, PA>(a, al, st, acc, o);
   } else if constexpr (NW > 1)
     gmf_lane<KIND, ORDER, PA>(a, al, bt, o, g_gmf_recip.r, GmfSeriesShared<KIND, ORDER, PA, NW>{bt, pool, tid});

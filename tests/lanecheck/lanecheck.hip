@@ -2,6 +2,7 @@
 // CPU, assembled into the API layouts the way the kernels assemble them, so that tests/test_lanecheck.py can compare
 // it with the oracle without a GPU.  Test infrastructure: built by __graft_entry__.build(), loaded only by tests.
 #include <stdint.h>
+#include <string.h>
 
 #include "gmf_device.hpp"
 #include "hencky_device.hpp"
@@ -133,7 +134,39 @@ void gmf_series_check(double al, int64_t n, const double* A, int* cls, double* e
   }
 }
 
+// the staged nodes (gmf_stage_nodes -> gmf_unstage_nodes) of each matrix's log order 2 nodes: nbad = matrices whose
+// nodes do not come back bitwise (bit patterns, so the signs of zeros count); ncplx = matrices stored with their
+// imaginary parts
+void gmf_stage_check(int64_t n, const double* A, int64_t* nbad, int64_t* ncplx) {
+  mf::GmfCoef bt;
+  mf::gmf_coefs(mf::LOG, 0.0, bt);
+  static constexpr mf::GmfRecip rinv = mf::gmf_recip_table();
+  *nbad = *ncplx = 0;
+  for (int64_t m = 0; m < n; m++) {
+    mf::GmfNodes st;
+    mf::GmfSeriesIn s;
+    mf::gmf_nodes<mf::LOG, 2, false>(A + 9 * m, 0.0, bt.b, rinv.r, st, s);
+    mf::GmfNodesRe r;
+    double im[6] = {7.0, 7.0, 7.0, 7.0, 7.0, 7.0};
+    mf::gmf_stage_nodes(st, mf::gmf_series_class(s), r, im);
+    const mf::GmfNodes b = mf::gmf_unstage_nodes(r, im);
+    *ncplx += (r.flags & mf::GMF_NODES_CPLX) ? 1 : 0;
+    bool ok = ((r.flags >> 9) & 3u) == (unsigned)mf::gmf_series_class(s) && b.flags == st.flags;
+    const double va[16] = {st.x[0].re, st.x[0].im, st.x[1].re, st.x[1].im, st.x[2].re, st.x[2].im, st.mu[0].re,
+                           st.mu[0].im, st.mu[1].re, st.mu[1].im, st.mu[2].re, st.mu[2].im, st.m, st.sig, st.xshift,
+                           st.cxg};
+    const double vb[16] = {b.x[0].re, b.x[0].im, b.x[1].re, b.x[1].im, b.x[2].re, b.x[2].im, b.mu[0].re,
+                           b.mu[0].im, b.mu[1].re, b.mu[1].im, b.mu[2].re, b.mu[2].im, b.m, b.sig, b.xshift, b.cxg};
+    ok = ok && memcmp(va, vb, sizeof(va)) == 0 && memcmp(&st.L, &b.L, sizeof(double)) == 0;
+    *nbad += ok ? 0 : 1;
+  }
+}
+
 }  // namespace
+
+extern "C" void lanecheck_gmf_stage(int64_t n, const double* A, int64_t* nbad, int64_t* ncplx) {
+  gmf_stage_check(n, A, nbad, ncplx);
+}
 
 extern "C" int lanecheck_gmf_series_real(int kind, double al, int order, int64_t n, const double* A, int* cls,
                                          double* err) {

@@ -238,3 +238,17 @@ def test_gmf_real_series_loops_equal_the_complex_loop(lane, kind, alpha):
         assert err[real, 2].max() == 0.0, (kind, order)
         assert err[cls == 0, 0].max() <= 1e-14, (kind, order, err[cls == 0, 0].max())
         assert err[real, 1].max() <= 1e-14, (kind, order, err[real, 1].max())
+
+
+def test_gmf_staged_nodes_come_back_bitwise(lane):
+    """the staged evaluation keeps the nodes in a real record plus the imaginary parts only where one is not +0
+    (gmf_stage_nodes / gmf_unstage_nodes): every matrix of the general mix, and matrices with a negative-zero or NaN
+    entry, get their nodes back bit for bit, with the series class in the flags"""
+    lane.lanecheck_gmf_stage.argtypes = [ctypes.c_int64, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]
+    A = np.ascontiguousarray(W.general(20000, seed=42))
+    A[5] = [-0.0, 0.0, 0.0, 0.0, 2.0, 0.0, 0.0, 0.0, 3.0]
+    A[6] = [np.nan, 0.0, 0.0, 0.0, 2.0, 0.0, 0.0, 0.0, 3.0]
+    nbad, ncplx = ctypes.c_int64(), ctypes.c_int64()
+    lane.lanecheck_gmf_stage(len(A), A.ctypes.data, ctypes.byref(nbad), ctypes.byref(ncplx))
+    assert nbad.value == 0
+    assert 1000 < ncplx.value < len(A) - 1000, ncplx.value          # (both layouts occur)
