@@ -551,6 +551,14 @@ def main():
     if not (args.only_headline or args.only_hencky):
         legs["smf_mixed"] = leg("smf mixed", lib.smf_hip_eval_matrix_mixed, dA, n, 4, args.steps, args.warmup,
                                 funcs_mx)
+        # the mixed form's own ceiling: the FP32 output shape after the same input reads (3348 B per matrix)
+        probe_in32 = lambda: lib.mf_hip_store_probe_in_f32(n, dA.data_ptr(), 9, 9, Fb.data_ptr(), Jb.data_ptr(),
+                                                           Hb.data_ptr(), sh)
+        pm, _ = timed(torch, stream, [probe_in32], max(args.steps, 5), args.warmup)
+        legs["smf_mixed"]["io_ceiling"] = {"probe_in_ms": pm[0], "probe_in_GBs": n * 3348 / (pm[0] * 1e-3) / 1e9,
+                                           "frac_of_io_ceiling": pm[0] / legs["smf_mixed"]["avg_kernel_ms"]}
+        log(f"bench: smf mixed against its ceiling (FP32 store probe with the input reads): {pm[0]:.3f} ms, kernel at "
+            f"{legs['smf_mixed']['io_ceiling']['frac_of_io_ceiling']:.1%} of it")
         # packed symmetric outputs (F 6, J 21, H 126 per matrix) of the same matrices as Voigt vectors
         dV = torch.from_numpy(W.voigt(A)).to(dev)
         legs["smf_packed_fp64"] = leg("smf packed fp64", lib.smf_hip_eval_packed, dV, n, 8, args.steps, args.warmup,
