@@ -108,13 +108,25 @@ def cpu_baseline(A, seconds=10.0, chunk=1 << 17):
 
 
 def cpu_model():
+    """the host CPU as /proc/cpuinfo names it, with its family / model / stepping numbers and whether it has AVX-512
+    (virtual machines often report only the vendor line as the model name)"""
+    info = {}
     try:
         for line in open("/proc/cpuinfo"):
-            if line.startswith("model name"):
-                return line.split(":", 1)[1].strip()
+            if ":" in line:
+                k, v = (x.strip() for x in line.split(":", 1))
+                if k in ("model name", "cpu family", "model", "stepping", "flags") and k not in info:
+                    info[k] = v
+            if not line.strip() and info:
+                break
     except OSError:
-        pass
-    return "unknown"
+        return "unknown"
+    name = info.get("model name", "unknown")
+    extra = [f"family {info['cpu family']}" if "cpu family" in info else "",
+             f"model {info['model']}" if "model" in info else "",
+             f"stepping {info['stepping']}" if "stepping" in info else "",
+             "AVX-512" if "avx512f" in info.get("flags", "").split() else "no AVX-512"]
+    return f"{name} ({', '.join(e for e in extra if e)})"
 
 
 def c0_baseline(runs=7, n=65536):
