@@ -278,9 +278,10 @@ MF_DEV int gmf_series_class(const GmfSeriesIn& s) {
 
 // gmf_series for a lane whose group offsets u are real: the dual vectors E and the sums stay real (their imaginary
 // parts are zero in gmf_series), so the recurrence runs in real arithmetic -- two FP64 operations per entry and term
-// instead of six.  POLE = false: no lane of the caller's wavefront has a pole node, so every entry takes b_j (none of
-// the factor selects; the lagged terms past K are not needed either).  The sums equal gmf_series' up to rounding (the
-// two loops contract into FMAs differently); a lane whose offsets are not real must not call it.
+// instead of six.  POLE = false: for a lane without a pole node (gmf_series_class 0), whose entries all take b_j
+// (none of the factor selects; the lagged terms past K are not needed either); POLE = true: class 1.  The sums equal
+// gmf_series' up to rounding (the loops contract into FMAs differently); a lane whose offsets are not real (class 2)
+// must not call it.
 template <int KIND, int ORDER, bool PA, bool POLE>
 MF_DEV void gmf_series_real(const GmfSeriesIn& s, const double* __restrict__ bt, cpx* acc) {
   constexpr int ND = ORDER == 0 ? 3 : (ORDER == 1 ? 9 : 19), LAG = ORDER + 1;

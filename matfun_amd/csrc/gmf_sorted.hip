@@ -71,10 +71,10 @@ __global__ void __launch_bounds__(KLEN_THREADS)
 }
 
 // the group series of the matrices in sorted order: lane t of the batch takes matrix perm[t] (its inputs from
-// sin, its ND sums at the matrix's own row: real parts to sre, imaginary parts to sim for series class 2 only).  The
-// keys order the series by class, so most wavefronts hold one class: a wavefront whose offsets are all real runs the
-// real loop (without the pole factors when no lane has a pole node), any other the complex loop (the choice is the
-// wavefront's, by ballot: no divergence)
+// sin, its ND sums at the matrix's own row: real parts to sre, imaginary parts to sim for series class 2 only).  Each
+// matrix runs its class's loop (the real loop without or with the pole factors, or the complex loop), so its sums do
+// not depend on which matrices share its wavefront (the order within a key is that of the sort's atomics); the keys
+// order the series by class, so a wavefront runs more than one loop only where the classes meet
 template <int KIND, int ORDER, bool PA>
 __global__ void __launch_bounds__(KLEN_THREADS)
     gmf_series_kernel(int64_t n, const int32_t* __restrict__ perm, const GmfSeriesIn* __restrict__ sin, GmfCoef bt,
@@ -86,8 +86,8 @@ __global__ void __launch_bounds__(KLEN_THREADS)
   const GmfSeriesIn s = sin[i];
   const int cls = gmf_series_class(s);
   cpx acc[ND];
-  if (__ballot(cls != 0) == 0) gmf_series_real<KIND, ORDER, PA, false>(s, bt.b, acc);
-  else if (__ballot(cls == 2) == 0) gmf_series_real<KIND, ORDER, PA, true>(s, bt.b, acc);
+  if (cls == 0) gmf_series_real<KIND, ORDER, PA, false>(s, bt.b, acc);
+  else if (cls == 1) gmf_series_real<KIND, ORDER, PA, true>(s, bt.b, acc);
   else gmf_series<KIND, ORDER, PA>(s, bt.b, acc);
 #pragma unroll
   for (int c = 0; c < ND; c++) sre[i * ND + c] = acc[c].re;

@@ -152,6 +152,33 @@ def test_staged_parts_equal_one_pass():
         lib.gmf_hip_set_sort_strategy(1, 2, 0, saved)
 
 
+def test_staged_results_do_not_depend_on_the_order_of_the_batch():
+    """the staged strategy's results per matrix are bitwise the same whatever the batch's order (the order within a
+    sort key comes from atomics, so it changes from run to run as well): the batch and a shuffled copy, every kind,
+    orders 1 and 2, FP64 and mixed"""
+    import matfun_amd
+    lib = matfun_amd.library()
+    G = W.general(6000, seed=706)
+    perm = np.random.default_rng(707).permutation(len(G))
+    keys = [(k, o, x) for k in range(5) for o in range(3) for x in range(2)]
+    saved = {key: lib.gmf_hip_get_sort_strategy(*key) for key in keys}
+    for key in keys:
+        assert lib.gmf_hip_set_sort_strategy(*key, 2) == 0
+    try:
+        for kind, alpha in KINDS:
+            for order in (1, 2):
+                for mixed in (False, True):
+                    print(f"staged, shuffled batch: {kind} {alpha} order {order} mixed {mixed}", flush=True)
+                    a = run(G, kind, alpha, order, mixed, True)
+                    b = run(G[perm], kind, alpha, order, mixed, True)
+                    for x, y in zip(a, b):
+                        if y is not None:
+                            assert np.array_equal(x[perm], y, equal_nan=True), (kind, alpha, order, mixed)
+    finally:
+        for key, v in saved.items():
+            lib.gmf_hip_set_sort_strategy(*key, v)
+
+
 def test_strategy_table_rejects_invalid_entries():
     import matfun_amd
     lib = matfun_amd.library()
