@@ -100,9 +100,11 @@ int gmf_hip_eval_pow_sorted_mixed(const double *alpha, int64_t n, const double *
  * hipGraph capture, tight serving loops, harnesses that forbid hipMalloc): no counterpart in the reference.  The
  * workspace is a static device array of the library (21.2 MB on each device the library runs on, held until the
  * process ends); nothing is allocated per call.  At most 2^22 matrices per call (-1 beyond: split the batch); the
- * same strategies, results (bitwise: a matrix's results do not depend on the parts a batch runs in), arguments and
- * return codes as gmf_hip_eval_sorted otherwise, FP64.  Calls that overlap on one device share the array and race:
- * issue them on one stream (or serialise them). */
+ * same strategies, arguments and return codes as gmf_hip_eval_sorted otherwise, FP64.  The array holds a staged
+ * evaluation of up to about 39000 matrices at order 2 (56000 at order 1) in one part, larger batches in parts of that
+ * size: the same results as gmf_hip_eval_sorted bitwise (a matrix's results do not depend on the parts a batch runs
+ * in or on its place in the batch).  Calls that overlap on one device share the array and race: issue them on one
+ * stream (or serialise them). */
 int gmf_hip_eval_sorted_noalloc(int kind, double alpha, int64_t n, const double *A, double *F, double *J, double *H,
                                 void *stream);
 /* the strategy of gmf_hip_eval_sorted* for a function (kind 0..3; 4 = per-matrix exponents), order (0..2) and
