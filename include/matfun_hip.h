@@ -167,18 +167,25 @@ int mf_hip_batch_eval(const mf_hip_plan *plan, const double *A, const double *al
 int mf_hip_batch_eval_mixed(const mf_hip_plan *plan, const double *A, const double *alpha, float *F, float *J,
                             float *H, void *stream);
 
-/* Launch strategy (every strategy gives bitwise the same results):
+/* Launch strategy (modes 0-2 give bitwise the same results):
  *   mode 0 fused: one kernel over the tiles of all classes, tiles in class order (order 0) or interleaved in
  *          proportion to the classes' sizes (order 1)
  *   mode 1 split: one kernel per class (each compiled for its class alone), one after another on `stream`
  *   mode 2 split, concurrent: the class kernels on `stream` and up to 3 streams of the plan (forked from and joined
  *          back to `stream` with events, so the call stays asynchronous and capturable; calls that share a plan
  *          must not overlap)
+ *   mode 3 split, staged: as mode 1, but the general-matrix log, sqrt and pow classes of order 1 and 2 run the
+ *          staged evaluation of mf_hip_gmf_eval_sorted (strategy 2) over their matrices: their results agree with
+ *          mode 1's to rounding (the series summed in another order; the same bound), the other classes' bitwise.
+ *          Uses a workspace of the plan (about 0.6 KB per matrix of the largest such class), allocated by this call
+ *          or by mf_hip_plan_tune; calls that share a plan must not overlap.  Returns -2 if the allocation fails.
  *   per_cu > 0: persistent, per_cu workgroups per compute unit loop over the tiles; 0: one workgroup per tile
- * mf_hip_plan_tune times the split candidates (modes 1 and 2, one workgroup per tile or 2, 4, 8 per CU; mode 0 is
- * measured slower on MI355X in every form -- its register use and LDS are those of the largest class for every tile --
- * and is chosen only through mf_hip_plan_set_strategy) on the caller's buffers (the batch is evaluated several times)
- * and keeps the fastest; `result` (optional) receives {mode, per_cu, order, microseconds of the best}.  Synchronous. */
+ *          (ignored by mode 3)
+ * mf_hip_plan_tune times the split candidates (modes 1 and 2, one workgroup per tile or 2, 4, 8 per CU, and mode 3
+ * when the batch has a staged class; mode 0 is measured slower on MI355X in every form -- its register use and LDS are
+ * those of the largest class for every tile -- and is chosen only through mf_hip_plan_set_strategy) on the caller's
+ * buffers (the batch is evaluated several times) and keeps the fastest; `result` (optional) receives {mode, per_cu,
+ * order, microseconds of the best}.  Synchronous. */
 int mf_hip_plan_set_strategy(mf_hip_plan *plan, int mode, int per_cu, int order);
 int mf_hip_plan_tune(mf_hip_plan *plan, const double *A, const double *alpha, void *F, void *J, void *H, int mixed,
                      void *stream, double *result);

@@ -13,6 +13,7 @@
 
 #include "../../include/matfun_hip.h"
 #include "batch_kernels.hpp"
+#include "gmf_sorted.hpp"
 #include "philox.hpp"
 
 namespace mf {
@@ -49,6 +50,11 @@ struct mf_hip_plan {
   BatchTile* d_mixed = nullptr;           // interleaved in proportion to the class sizes
   int64_t *d_rm = nullptr, *d_rj = nullptr, *d_rh = nullptr;
   double* d_alpha = nullptr;              // the groups' exponents per matrix (pow groups; 1 elsewhere)
+  BatchTables tab;                        // (host copy: the gmf series tables of the staged classes)
+  // mode 3: the workspace of the staged general-matrix classes (one class at a time on the caller's stream),
+  // allocated when the mode is first set
+  void* d_work = nullptr;
+  size_t work_bytes = 0;
   int mode = 0, per_cu = 6, order = 1;    // launch strategy
   // mode 2: the classes on the caller's stream and up to NSIDE side streams (default priority), forked and joined
   // with events; the classes' assignment balanced by estimated cost
@@ -79,7 +85,7 @@ int upload(T** d, const T* h, size_t count) {
 
 void free_plan(mf_hip_plan* p) {
   if (!p) return;
-  void* ptrs[] = {p->d_sorted, p->d_mixed, p->d_rm, p->d_rj, p->d_rh, p->d_alpha};
+  void* ptrs[] = {p->d_sorted, p->d_mixed, p->d_rm, p->d_rj, p->d_rh, p->d_alpha, p->d_work};
   for (void* q : ptrs)
     if (q) (void)hipFree(q);
   for (int k = 0; k < mf_hip_plan::NSIDE; k++) {
@@ -117,13 +123,50 @@ unsigned grid_for(const mf_hip_plan* p, int64_t ntiles, int per_cu) {
   return (unsigned)std::max<int64_t>(g, 1);
 }
 
-// mode 0: the kernel over all classes; mode 1: one kernel per class.  per_cu > 0: persistent, per_cu workgroups per
-// compute unit; 0: one workgroup per tile.  order (mode 0): 0 class-major tiles, 1 interleaved.
+// the matrices of class c (its tiles are full but the last)
+int64_t cls_matrices(const mf_hip_plan* p, int c) {
+  const int64_t t = p->cls_t1[c] - p->cls_t0[c];
+  return t == 0 ? 0 : (t - 1) * WAVE + p->tiles[(size_t)(p->cls_t1[c] - 1)].nm;
+}
+
+// the workspace bytes of mode 3: the largest stageable class's
+size_t staged_work_bytes(const mf_hip_plan* p) {
+  size_t b = 0;
+  for (int c = 0; c < NCLS; c++)
+    if (batch_gmf_stageable(c)) b = std::max(b, gmf_staged_list_bytes(cls_matrices(p, c), cls_order(c)));
+  return b;
+}
+
+// mode 0: the kernel over all classes; mode 1: one kernel per class; mode 3: as mode 1, the gmf classes of order >= 1
+// other than exp through the staged evaluation (gmf_staged_list, then their tiles finished from the staged data).
+// per_cu > 0: persistent, per_cu workgroups per compute unit; 0: one workgroup per tile.  order (mode 0): 0
+// class-major tiles, 1 interleaved.
 template <class OT>
 int batch_eval(const mf_hip_plan* p, const double* A, const double* alpha, OT* F, OT* J, OT* H, hipStream_t st,
                int mode, int per_cu, int order) {
   if (!p || (p->N > 0 && (!A || !F)) || (p->needJ && !J) || (p->needH && !H)) return -1;
   if (p->ntiles == 0) return 0;
+  if (mode == 3) {
+    if (!p->d_work && staged_work_bytes(p) > 0) return -1;   // (mf_hip_plan_set_strategy allocates it)
+    for (int c = 0; c < NCLS; c++) {
+      const int64_t n = p->cls_t1[c] - p->cls_t0[c];
+      if (n == 0) continue;
+      const BatchArgs a = args_of(p, p->d_sorted, p->cls_t0[c], p->cls_t1[c], A, alpha);
+      if (batch_gmf_stageable(c)) {
+        const int slot = cls_slot(c), ord = cls_order(c);
+        const int64_t off0 = p->tiles[(size_t)p->cls_t0[c]].off;
+        const GmfCoef& bt = slot == SLOT_POW ? p->tab.recips : p->tab.gmf[slot];
+        const GmfStaged sg = gmf_staged_list(st, slot == 1 ? LOG : POW, ord, slot == SLOT_POW, cls_matrices(p, c),
+                                             p->d_rm + off0, A, slot == 2 ? 0.5 : 1.0, a.alv, bt, p->d_work);
+        batch_launch_gmf_staged(c, st, (unsigned)n, a, off0, sg, F, J, H);
+      } else if (cls_fam(c) == 0) {
+        batch_launch_smf(c, st, grid_for(p, n, per_cu), per_cu > 0, a, F, J, H);
+      } else {
+        batch_launch_gmf(c, st, grid_for(p, n, per_cu), per_cu > 0, a, F, J, H);
+      }
+    }
+    return hipGetLastError() == hipSuccess ? 0 : -2;
+  }
   if (mode == 0) {
     const BatchArgs a = args_of(p, order == 1 ? p->d_mixed : p->d_sorted, 0, p->ntiles, A, alpha);
     batch_launch_fused(st, grid_for(p, p->ntiles, per_cu), per_cu > 0, a, F, J, H);
@@ -258,6 +301,7 @@ extern "C" int mf_hip_plan_create(const mf_hip_group* groups, int64_t ngroups, m
   gmf_coefs(LOG, 0.0, tab.gmf[1]);
   gmf_coefs(POW, 0.5, tab.gmf[2]);
   gmf_recips(tab.recips);
+  p->tab = tab;
   int dev = 0;
   if (hipGetDevice(&dev) != hipSuccess || hipDeviceGetAttribute(&p->cus, hipDeviceAttributeMultiprocessorCount, dev) !=
                                               hipSuccess) {
@@ -313,8 +357,22 @@ extern "C" int mf_hip_batch_eval_mixed(const mf_hip_plan* p, const double* A, co
   return p ? batch_eval(p, A, alpha, F, J, H, (hipStream_t)stream, p->mode, p->per_cu, p->order) : -1;
 }
 
+// the workspace of mode 3, allocated once per plan (0, or -2 when the allocation fails)
+static int alloc_staged_work(mf_hip_plan* p) {
+  if (p->d_work) return 0;
+  const size_t b = staged_work_bytes(p);
+  if (b == 0) return 0;
+  if (hipMalloc(&p->d_work, b) != hipSuccess) {
+    p->d_work = nullptr;
+    return -2;
+  }
+  p->work_bytes = b;
+  return 0;
+}
+
 extern "C" int mf_hip_plan_set_strategy(mf_hip_plan* p, int mode, int per_cu, int order) {
-  if (!p || mode < 0 || mode > 2 || per_cu < 0 || per_cu > 64 || order < 0 || order > 1) return -1;
+  if (!p || mode < 0 || mode > 3 || per_cu < 0 || per_cu > 64 || order < 0 || order > 1) return -1;
+  if (mode == 3 && alloc_staged_work(p) != 0) return -2;
   p->mode = mode;
   p->per_cu = per_cu;
   p->order = order;
@@ -333,6 +391,7 @@ extern "C" int mf_hip_plan_tune(mf_hip_plan* p, const double* A, const double* a
   std::vector<Cand> cands;
   for (int k : {0, 2, 4, 8}) cands.push_back({1, k, 0});
   for (int k : {0, 2}) cands.push_back({2, k, 0});
+  if (staged_work_bytes(p) > 0 && alloc_staged_work(p) == 0) cands.push_back({3, 0, 0});
   hipStream_t st = (hipStream_t)stream;
   hipEvent_t e0, e1;
   if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess) return -2;

@@ -60,6 +60,13 @@ void batch_launch_gmf(int cls, hipStream_t st, unsigned grid, bool loop, const B
                       double* H);
 void batch_launch_gmf(int cls, hipStream_t st, unsigned grid, bool loop, const BatchArgs& a, float* F, float* J,
                       float* H);
+// a gmf class of order >= 1 (not exp) finished from staged data (gmf_staged_list over the class's matrices in tile
+// order: matrix k of the class at k = tile offset - off0 + place in the tile), one workgroup per tile
+bool batch_gmf_stageable(int cls);
+void batch_launch_gmf_staged(int cls, hipStream_t st, unsigned grid, const BatchArgs& a, int64_t off0,
+                             const GmfStaged& sg, double* F, double* J, double* H);
+void batch_launch_gmf_staged(int cls, hipStream_t st, unsigned grid, const BatchArgs& a, int64_t off0,
+                             const GmfStaged& sg, float* F, float* J, float* H);
 
 #ifdef MF_BATCH_KERNELS
 // constant memory: uniform reads of it are scalar loads, so the tables take no vector registers
@@ -169,6 +176,36 @@ batch_class_kernel(BatchArgs a, OT* __restrict__ F, OT* __restrict__ J, OT* __re
       run_tile<CLS, OT>(a, tl, ListRows{rows, rows + WAVE, rows + 2 * WAVE}, F, J, H, lds, lutJ, lutH);
     }
     if (!LOOP) break;
+  }
+}
+
+// one tile of a gmf class from staged data (the finishing pass of the staged evaluation over the tile's listed rows)
+template <int CLS, class OT>
+__global__ void __launch_bounds__(WAVE) __attribute__((amdgpu_waves_per_eu(2)))
+batch_staged_kernel(BatchArgs a, int64_t off0, GmfStaged sg, OT* __restrict__ F, OT* __restrict__ J,
+                    OT* __restrict__ H) {
+  constexpr int SLOT = cls_slot(CLS), ORDER = cls_order(CLS);
+  constexpr int KIND = SLOT == 0 ? EXP : (SLOT == 1 ? LOG : POW);
+  static_assert(cls_fam(CLS) == 1 && ORDER >= 1, "gmf classes of order >= 1");
+  __shared__ __attribute__((aligned(16))) unsigned char lds[GmfLds<ORDER, OT>::BYTES];
+  constexpr bool R32 = ORDER == 2 && sizeof(OT) == 8;
+  __shared__ int64_t rows[R32 ? 1 : 3 * WAVE];
+  __shared__ int32_t rows32[R32 ? WAVE : 1];
+  const int lane = threadIdx.x;
+  const int64_t t = a.t0 + blockIdx.x;
+  if (t >= a.t1) return;
+  const BatchTile tl = a.tiles[t];
+  const double* bt = SLOT == SLOT_POW ? g_tab.recips.b : g_tab.gmf[SLOT].b;
+  const GmfStaged s{sg.nre, sg.nim, sg.sre, sg.sim, 0, tl.off - off0};
+  if constexpr (R32) {
+    gmf_tile<KIND, ORDER, OT, SLOT == SLOT_POW, ListRows32, 1, true>(stage_rows32(a, tl, rows32, lane), tl.nm, a.A, F,
+                                                                     J, H, SLOT == 2 ? 0.5 : 0.0, a.alv, bt, lds,
+                                                                     lane, nullptr, 0, s);
+  } else {
+    stage_rows(a, tl, rows, lane);
+    gmf_tile<KIND, ORDER, OT, SLOT == SLOT_POW, ListRows, 1, true>(ListRows{rows, rows + WAVE, rows + 2 * WAVE}, tl.nm,
+                                                                   a.A, F, J, H, SLOT == 2 ? 0.5 : 0.0, a.alv, bt,
+                                                                   lds, lane, nullptr, 0, s);
   }
 }
 

@@ -50,14 +50,16 @@ __global__ void __launch_bounds__(KLEN_THREADS)
   key[i] = (uint8_t)(GMF_KMAX - o.K);
 }
 
-// the series inputs and the key of matrices [0, n) (one lane per matrix; the first pass of the staged evaluation)
+// the series inputs and the key of matrices k = 0 .. n-1 (one lane per matrix; the first pass of the staged
+// evaluation): matrix k is row rows[k] of A (and of alv) when rows is given, else row k; outputs at k
 template <int KIND, int ORDER, bool PA>
 __global__ void __launch_bounds__(KLEN_THREADS)
-    gmf_prep_kernel(int64_t n, const double* __restrict__ A, double al, const double* __restrict__ alv, GmfCoef bt,
-                    uint8_t* __restrict__ key, GmfSeriesIn* __restrict__ sin, GmfNodesRe* __restrict__ nre,
-                    double* __restrict__ nim) {
-  const int64_t i = (int64_t)blockIdx.x * KLEN_THREADS + threadIdx.x;
-  if (i >= n) return;
+    gmf_prep_kernel(int64_t n, const int64_t* __restrict__ rows, const double* __restrict__ A, double al,
+                    const double* __restrict__ alv, GmfCoef bt, uint8_t* __restrict__ key, GmfSeriesIn* __restrict__ sin,
+                    GmfNodesRe* __restrict__ nre, double* __restrict__ nim) {
+  const int64_t k = (int64_t)blockIdx.x * KLEN_THREADS + threadIdx.x;
+  if (k >= n) return;
+  const int64_t i = rows ? rows[k] : k;
   double a[9];
 #pragma unroll
   for (int k = 0; k < 9; k++) a[k] = A[9 * i + k];
@@ -66,8 +68,8 @@ __global__ void __launch_bounds__(KLEN_THREADS)
     al = fabs(al) <= 1.7976931348623157e308 ? al : 1.0;
   }
   GmfOut o;
-  gmf_lane<KIND, ORDER, PA>(a, al, bt.b, o, g_gmf_recip.r, GmfSeriesLength{sin + i, nre + i, nim + 6 * i});
-  key[i] = (uint8_t)(o.cls == 0 ? GMF_KMAX - o.K : GMF_KMAX + 1 + (o.cls - 1) * KQ + (GMF_KMAX - o.K) / 8);
+  gmf_lane<KIND, ORDER, PA>(a, al, bt.b, o, g_gmf_recip.r, GmfSeriesLength{sin + k, nre + k, nim + 6 * k});
+  key[k] = (uint8_t)(o.cls == 0 ? GMF_KMAX - o.K : GMF_KMAX + 1 + (o.cls - 1) * KQ + (GMF_KMAX - o.K) / 8);
 }
 
 // the group series of the matrices in sorted order: lane t of the batch takes matrix perm[t] (its inputs from
@@ -273,9 +275,11 @@ size_t staged_bytes(int64_t m, int order) {
          align256(6 * sizeof(double) * (size_t)m) + 2 * align256(sizeof(double) * nd * (size_t)m);
 }
 
-template <int KIND, int ORDER, class OT, bool PA>
-void staged_order(hipStream_t st, int64_t m, const double* A, OT* F, OT* J, OT* H, double al, const double* alv,
-                  const GmfCoef& bt, void* work) {
+// the passes before the finishing pass (nodes and series inputs, the sort, the series) of m matrices, matrix k being
+// row rows[k] of A (or row k): the staged data, indexed by k, in the workspace (staged_bytes(m, ORDER))
+template <int KIND, int ORDER, bool PA>
+GmfStaged staged_front(hipStream_t st, int64_t m, const int64_t* rows, const double* A, double al, const double* alv,
+                       const GmfCoef& bt, void* work) {
   const SortSpace s = carve(work, m);
   unsigned char* p = reinterpret_cast<unsigned char*>(s.off) + align256(4 * (size_t)KSORT_MAXCH * KBINS);
   GmfSeriesIn* sin = reinterpret_cast<GmfSeriesIn*>(p);
@@ -292,15 +296,22 @@ void staged_order(hipStream_t st, int64_t m, const double* A, OT* F, OT* J, OT* 
   chunk = (chunk + SORT_THREADS - 1) / SORT_THREADS * SORT_THREADS;
   const int nch = (int)((m + chunk - 1) / chunk);
   const unsigned gl = (unsigned)((m + KLEN_THREADS - 1) / KLEN_THREADS);
-  hipLaunchKernelGGL((gmf_prep_kernel<KIND, ORDER, PA>), dim3(gl), dim3(KLEN_THREADS), 0, st, m, A, al, alv, bt, s.key,
-                     sin, nre, nim);
+  hipLaunchKernelGGL((gmf_prep_kernel<KIND, ORDER, PA>), dim3(gl), dim3(KLEN_THREADS), 0, st, m, rows, A, al, alv, bt,
+                     s.key, sin, nre, nim);
   hipLaunchKernelGGL(key_hist_kernel, dim3(nch), dim3(SORT_THREADS), 0, st, m, chunk, s.key, s.hist);
   hipLaunchKernelGGL(key_scan_kernel, dim3(1), dim3(SORT_THREADS), 0, st, nch, s.hist, s.off);
   hipLaunchKernelGGL(key_scatter_kernel, dim3(nch), dim3(SORT_THREADS), 0, st, m, chunk, s.key, s.off, s.perm);
   hipLaunchKernelGGL((gmf_series_kernel<KIND, ORDER, PA>), dim3(gl), dim3(KLEN_THREADS), 0, st, m, s.perm, sin, bt,
                      sre, sim);
+  return GmfStaged{nre, nim, sre, sim, 0, -1};
+}
+
+template <int KIND, int ORDER, class OT, bool PA>
+void staged_order(hipStream_t st, int64_t m, const double* A, OT* F, OT* J, OT* H, double al, const double* alv,
+                  const GmfCoef& bt, void* work) {
+  const GmfStaged sg = staged_front<KIND, ORDER, PA>(st, m, nullptr, A, al, alv, bt, work);
   hipLaunchKernelGGL((gmf_finish_kernel<KIND, ORDER, OT, PA>), dim3((unsigned)((m + WAVE - 1) / WAVE)), dim3(WAVE), 0,
-                     st, m, A, F, J, H, al, alv, bt, GmfStaged{nre, nim, sre, sim});
+                     st, m, A, F, J, H, al, alv, bt, sg);
 }
 
 template <int KIND, class OT, bool PA>
@@ -362,6 +373,21 @@ void plain_launch(hipStream_t st, int slot, int kind, int64_t n, const double* A
 }
 
 }  // namespace
+
+size_t gmf_staged_list_bytes(int64_t m, int order) { return m > 0 ? staged_bytes(m, order) : 0; }
+
+GmfStaged gmf_staged_list(hipStream_t st, int kind, int order, bool pa, int64_t m, const int64_t* rows,
+                          const double* A, double al, const double* alv, const GmfCoef& bt, void* work) {
+#define MF_FRONT(K, P)                                                                       \
+  if (order == 2) return staged_front<K, 2, P>(st, m, rows, A, al, alv, bt, work);          \
+  if (order == 1) return staged_front<K, 1, P>(st, m, rows, A, al, alv, bt, work);          \
+  return staged_front<K, 0, P>(st, m, rows, A, al, alv, bt, work);
+  if (pa) { MF_FRONT(POW, true) }
+  if (kind == EXP) { MF_FRONT(EXP, false) }
+  if (kind == LOG) { MF_FRONT(LOG, false) }
+  MF_FRONT(POW, false)
+#undef MF_FRONT
+}
 
 int gmf_sort_strategy_get(int slot, int order, bool mixed) {
   init_strategies();

@@ -102,14 +102,16 @@ struct GmfLds {
 // its own series; NW > 1: the tile is one of NW in a workgroup of NW wavefronts (every one calling gmf_tile once) that
 // share out the series (GmfSeriesShared; pool: its LDS, tid = threadIdx.x).  Ends with the tile's outputs written and
 // a wave barrier (the LDS may be reused).
-// the nodes and group series sums of each matrix computed by earlier passes (the staged evaluation, gmf_sorted.hip):
-// nodes nre[row] and nim[6 row ..] (gmf_stage_nodes), the real parts of the sums sre[row ND ..], their imaginary
-// parts sim[row ND ..] for a matrix of series class 2 (else +0)
+// the nodes and group series sums of each matrix computed by earlier passes (the staged evaluation, gmf_sorted.hip),
+// at index k: nodes nre[k] and nim[6 k ..] (gmf_stage_nodes), the real parts of the sums sre[k ND ..], their
+// imaginary parts sim[k ND ..] for a matrix of series class 2 (else +0).  k = row - off for the tile matrix's input
+// row, or, with pos0 >= 0, pos0 + its place in the tile (a batch class's matrices staged in tile order)
 struct GmfStaged {
   const GmfNodesRe* nre;
   const double* nim;
   const double* sre;
   const double* sim;
+  int64_t off, pos0;
 };
 
 // STAGED: each lane finishes its matrix from staged nodes and sums (gmf_finish) instead of running gmf_lane
@@ -117,7 +119,7 @@ template <int KIND, int ORDER, class OT, bool PA, class R, int NW = 1, bool STAG
 MF_DEV void gmf_tile(const R& rows, int nm, const double* __restrict__ A, OT* __restrict__ F, OT* __restrict__ J,
                      OT* __restrict__ H, double al, const double* __restrict__ alv, const double* __restrict__ bt,
                      unsigned char* lds, int lane, unsigned char* pool = nullptr, int tid = 0,
-                     GmfStaged staged = GmfStaged{nullptr, nullptr, nullptr, nullptr}) {
+                     GmfStaged staged = GmfStaged{nullptr, nullptr, nullptr, nullptr, 0, -1}) {
   using L = GmfLds<ORDER, OT>;
   constexpr int VEC = 16 / (int)sizeof(OT);             // (elements per 16-byte store)
   double* xin = reinterpret_cast<double*>(lds);
@@ -141,7 +143,8 @@ MF_DEV void gmf_tile(const R& rows, int nm, const double* __restrict__ A, OT* __
   GmfOut o;
   if constexpr (STAGED) {
     constexpr int ND = ORDER == 0 ? 3 : (ORDER == 1 ? 9 : 19);
-    const int64_t row = rows.m(lane < nm ? lane : 0);
+    const int ln = lane < nm ? lane : 0;
+    const int64_t row = staged.pos0 >= 0 ? staged.pos0 + ln : rows.m(ln) - staged.off;
     const GmfNodesRe nr = staged.nre[row];
     const GmfNodes st = gmf_unstage_nodes(nr, staged.nim + 6 * row);
     cpx acc[ND];

@@ -57,3 +57,33 @@ def call_pow(family, alphas, X, order, mixed=False):
     torch.cuda.synchronize()
     g = lambda t: None if t is None else t.cpu().double().numpy()
     return rc, g(dF), g(dJ), g(dH)
+
+
+def call_sorted(kind, alpha, X, order, mixed=False, alphas=None):
+    """run gmf_hip_eval_sorted / gmf_hip_eval_pow_sorted (alphas: one exponent per matrix), or their _mixed forms, on
+    host array X (n, 9) with a workspace of gmf_hip_sort_workspace(n) bytes, at the strategy the library's table
+    holds; returns rc, F, J, H (NumPy)"""
+    import torch
+    lib = matfun_amd.library()
+    dt = torch.float32 if mixed else torch.float64
+    X = np.ascontiguousarray(X, dtype=np.float64).reshape(-1, 9)
+    n = X.shape[0]
+    dX = torch.from_numpy(X).to("cuda")
+    dF = torch.full((n, 9), 7.0, dtype=dt, device="cuda")
+    dJ = torch.full((n, 81), 7.0, dtype=dt, device="cuda") if order >= 1 else None
+    dH = torch.full((n, 729), 7.0, dtype=dt, device="cuda") if order >= 2 else None
+    p = lambda t: None if t is None else t.data_ptr()
+    nb = lib.gmf_hip_sort_workspace(n)
+    work = torch.empty(max(nb, 1), dtype=torch.uint8, device="cuda")
+    st = torch.cuda.current_stream().cuda_stream
+    if alphas is not None:
+        dA = torch.from_numpy(np.ascontiguousarray(alphas, dtype=np.float64)).to("cuda")
+        fn = lib.gmf_hip_eval_pow_sorted_mixed if mixed else lib.gmf_hip_eval_pow_sorted
+        rc = fn(p(dA), n, p(dX), p(dF), p(dJ), p(dH), work.data_ptr(), nb, st)
+    else:
+        fn = lib.gmf_hip_eval_sorted_mixed if mixed else lib.gmf_hip_eval_sorted
+        kind = matfun_amd.KINDS[kind] if isinstance(kind, str) else int(kind)
+        rc = fn(kind, float(alpha), n, p(dX), p(dF), p(dJ), p(dH), work.data_ptr(), nb, st)
+    torch.cuda.synchronize()
+    g = lambda t: None if t is None else t.cpu().double().numpy()
+    return rc, g(dF), g(dJ), g(dH)

@@ -43,9 +43,15 @@ def inputs(groups, seed):
     return A
 
 
-def reference(groups, A, alpha, mixed):
-    """the homogeneous calls group by group; returns F, J, H stacked in the batch layout"""
-    from tests.gpu_util import call, call_pow
+def staged_group(fam, kind, order):
+    """the groups that batch strategy 3 evaluates through the staged general-matrix passes"""
+    return fam == "gmf" and kind != "exp" and order >= 1
+
+
+def reference(groups, A, alpha, mixed, staged=False):
+    """the homogeneous calls group by group; returns F, J, H stacked in the batch layout.  staged: the groups of
+    staged_group through gmf_hip_eval_sorted / gmf_hip_eval_pow_sorted (at the strategy the library's table holds)"""
+    from tests.gpu_util import call, call_pow, call_sorted
     Fs, Js, Hs = [], [], []
     lo = 0
     for fam, kind, order, count, al in groups:
@@ -53,8 +59,12 @@ def reference(groups, A, alpha, mixed):
             continue
         X = A[lo:lo + count]
         family = "matrix" if fam == "smf" else "general"
+        a = None
         if kind == "pow":
             a = alpha[lo:lo + count] if alpha is not None else np.full(count, al)
+        if staged and staged_group(fam, kind, order):
+            rc, F, J, H = call_sorted(kind, 0.0, X, order, mixed=mixed, alphas=a)
+        elif kind == "pow":
             rc, F, J, H = call_pow(family, a, X, order, mixed=mixed)
         else:
             rc, F, J, H = call(family, kind, 0.0, X, order, mixed=mixed)
@@ -118,6 +128,40 @@ def test_every_strategy_gives_the_same_results():
     b.close()
 
 
+@pytest.fixture
+def staged_sorted():
+    """every entry of gmf_hip_eval_sorted's strategy table set to the staged strategy (2), restored afterwards"""
+    import matfun_amd
+    lib = matfun_amd.library()
+    keys = [(k, o, x) for k in range(5) for o in range(3) for x in range(2)]
+    saved = {key: lib.gmf_hip_get_sort_strategy(*key) for key in keys}
+    for key in keys:
+        assert lib.gmf_hip_set_sort_strategy(*key, 2) == 0
+    yield
+    for key, v in saved.items():
+        lib.gmf_hip_set_sort_strategy(*key, v)
+
+
+@pytest.mark.parametrize("mixed", [False, True])
+@pytest.mark.parametrize("per_matrix_alpha", [False, True])
+def test_staged_strategy_equals_the_staged_sorted_calls(mixed, per_matrix_alpha, staged_sorted):
+    """strategy 3: the general-matrix log / sqrt / pow classes of order >= 1 bitwise gmf_hip_eval_sorted's staged
+    evaluation of the same matrices (each matrix's result does not depend on the others in its call), every other
+    class bitwise the homogeneous calls; groups spread over several tiles of one class among them"""
+    import matfun_amd
+    gs = groups_all_classes(612) + [("gmf", k, o, 150, 0.4) for k in ("log", "pow") for o in (1, 2)]
+    A = inputs(gs, 613)
+    alpha = np.random.default_rng(614).uniform(-1.0, 1.0, len(A)) if per_matrix_alpha else None
+    b = matfun_amd.Batch(gs)
+    b.set_strategy(3, 0, 0)
+    print(f"staged batch n={b.n} tiles={b.ntiles} mixed={mixed} alpha={per_matrix_alpha}", flush=True)
+    got = run_batch(b, A, alpha, mixed)
+    ref = reference(gs, A, alpha, mixed, staged=True)
+    for name, x, y in zip("FJH", got, ref):
+        assert same(x, y), (name, x.shape, y.shape, np.nanmax(np.abs(x - y)) if x.shape == y.shape else None)
+    b.close()
+
+
 def test_tuner_keeps_results_and_picks_a_strategy():
     import torch
     import matfun_amd
@@ -128,10 +172,11 @@ def test_tuner_keeps_results_and_picks_a_strategy():
     dA = torch.from_numpy(A).cuda()
     res = b.tune(dA)
     print(f"tuned: {res}", flush=True)
-    assert res["mode"] in (1, 2) and res["us"] > 0     # (the fused kernel, mode 0, is not a tuning candidate)
+    assert res["mode"] in (1, 2, 3) and res["us"] > 0     # (the fused kernel, mode 0, is not a tuning candidate)
     after = run_batch(b, A, None, False)
-    for x, y in zip(before, after):
-        assert same(x, y)
+    if res["mode"] != 3:             # (strategy 3 agrees with the others to rounding; its own test pins it)
+        for x, y in zip(before, after):
+            assert same(x, y)
     b.close()
 
 
@@ -158,9 +203,10 @@ def test_perturbation_matches_cpu_build_and_keeps_symmetry():
     b.close()
 
 
-@pytest.mark.parametrize("mixed,mode", [(False, 0), (True, 0), (False, 2)])
+@pytest.mark.parametrize("mixed,mode", [(False, 0), (True, 0), (False, 2), (False, 3), (True, 3)])
 def test_graph_sequence_equals_eager_steps(mixed, mode):
-    """mode 2 forks the class kernels onto the plan's streams: the capture follows the fork and the join"""
+    """mode 2 forks the class kernels onto the plan's streams: the capture follows the fork and the join; mode 3 runs
+    the staged classes' passes through the plan's workspace"""
     import torch
     import matfun_amd
     gs = groups_all_classes(610)
