@@ -100,8 +100,8 @@ MF_DEV void stage_rows(const BatchArgs& a, const BatchTile& tl, int64_t* rows, i
   __syncthreads();
 }
 
-// ListRows32 of a tile: its input rows into LDS (32-bit: a plan holds fewer than 2^31 matrices), the run masks of
-// the three regions by ballot
+// ListRows32 of a tile: its input rows into LDS (32-bit: a plan holds fewer than 2^31 matrices), its J and H rows
+// in registers, the run masks of the three regions by ballot
 MF_DEV ListRows32 stage_rows32(const BatchArgs& a, const BatchTile& tl, int32_t* rows, int lane) {
   const bool in = lane < tl.nm;
   int64_t m = 0, jr = 0, hr = 0;
@@ -113,7 +113,7 @@ MF_DEV ListRows32 stage_rows32(const BatchArgs& a, const BatchTile& tl, int32_t*
   }
   const int64_t pm = __shfl_up(m, 1), pj = __shfl_up(jr, 1), ph = __shfl_up(hr, 1);
   const bool on = in && lane > 0;
-  const ListRows32 R{rows, a.rj + tl.off, a.rh + tl.off,
+  const ListRows32 R{rows, jr, hr, 0,
                      {__ballot(on && m == pm + 1), __ballot(on && jr == pj + 1), __ballot(on && hr == ph + 1)}};
   __syncthreads();
   return R;
@@ -160,21 +160,13 @@ batch_class_kernel(BatchArgs a, OT* __restrict__ F, OT* __restrict__ J, OT* __re
   constexpr int B = FAM == 0 ? SmfLds<true, ORDER, OT>::BYTES : GmfLds<ORDER, OT>::BYTES;
   __shared__ __attribute__((aligned(16))) unsigned char lds[B];
   __shared__ short lutJ[FAM == 0 && ORDER >= 1 ? 81 + LUT_PAD : 1], lutH[FAM == 0 && ORDER >= 2 ? 729 + LUT_PAD : 1];
-  // the FP64 gmf order-2 tile leaves 304 B of LDS under the eight-workgroups-per-CU limit (20176 of 20480 B):
-  // its rows are staged as ListRows32 (256 B), the other classes' as ListRows (1.5 KB)
-  constexpr bool R32 = FAM == 1 && ORDER == 2 && sizeof(OT) == 8;
-  __shared__ int64_t rows[R32 ? 1 : 3 * WAVE];
-  __shared__ int32_t rows32[R32 ? WAVE : 1];
+  // the rows staged as ListRows32: 256 B of LDS (input rows), J and H rows in registers
+  __shared__ int32_t rows32[WAVE];
   const int lane = threadIdx.x;
   if (FAM == 0) smf_luts<true, ORDER, OT>(lutJ, lutH, lane);
   for (int64_t t = a.t0 + blockIdx.x; t < a.t1; t += gridDim.x) {
     const BatchTile tl = a.tiles[t];
-    if constexpr (R32) {
-      run_tile<CLS, OT>(a, tl, stage_rows32(a, tl, rows32, lane), F, J, H, lds, lutJ, lutH);
-    } else {
-      stage_rows(a, tl, rows, lane);
-      run_tile<CLS, OT>(a, tl, ListRows{rows, rows + WAVE, rows + 2 * WAVE}, F, J, H, lds, lutJ, lutH);
-    }
+    run_tile<CLS, OT>(a, tl, stage_rows32(a, tl, rows32, lane), F, J, H, lds, lutJ, lutH);
     if (!LOOP) break;
   }
 }
@@ -188,25 +180,16 @@ batch_staged_kernel(BatchArgs a, int64_t off0, GmfStaged sg, OT* __restrict__ F,
   constexpr int KIND = SLOT == 0 ? EXP : (SLOT == 1 ? LOG : POW);
   static_assert(cls_fam(CLS) == 1 && ORDER >= 1, "gmf classes of order >= 1");
   __shared__ __attribute__((aligned(16))) unsigned char lds[GmfLds<ORDER, OT>::BYTES];
-  constexpr bool R32 = ORDER == 2 && sizeof(OT) == 8;
-  __shared__ int64_t rows[R32 ? 1 : 3 * WAVE];
-  __shared__ int32_t rows32[R32 ? WAVE : 1];
+  __shared__ int32_t rows32[WAVE];
   const int lane = threadIdx.x;
   const int64_t t = a.t0 + blockIdx.x;
   if (t >= a.t1) return;
   const BatchTile tl = a.tiles[t];
   const double* bt = SLOT == SLOT_POW ? g_tab.recips.b : g_tab.gmf[SLOT].b;
   const GmfStaged s{sg.nre, sg.nim, sg.sre, sg.sim, 0, tl.off - off0};
-  if constexpr (R32) {
-    gmf_tile<KIND, ORDER, OT, SLOT == SLOT_POW, ListRows32, 1, true>(stage_rows32(a, tl, rows32, lane), tl.nm, a.A, F,
-                                                                     J, H, SLOT == 2 ? 0.5 : 0.0, a.alv, bt, lds,
-                                                                     lane, nullptr, 0, s);
-  } else {
-    stage_rows(a, tl, rows, lane);
-    gmf_tile<KIND, ORDER, OT, SLOT == SLOT_POW, ListRows, 1, true>(ListRows{rows, rows + WAVE, rows + 2 * WAVE}, tl.nm,
-                                                                   a.A, F, J, H, SLOT == 2 ? 0.5 : 0.0, a.alv, bt,
-                                                                   lds, lane, nullptr, 0, s);
-  }
+  gmf_tile<KIND, ORDER, OT, SLOT == SLOT_POW, ListRows32, 1, true>(stage_rows32(a, tl, rows32, lane), tl.nm, a.A, F, J,
+                                                                   H, SLOT == 2 ? 0.5 : 0.0, a.alv, bt, lds, lane,
+                                                                   nullptr, 0, s);
 }
 
 template <int C, int C1, class OT>

@@ -31,17 +31,18 @@ struct ListRows {
 };
 
 // rows listed per matrix with less LDS (for a tile body whose own LDS leaves no room for three 64-bit rows per
-// matrix): input / F rows 32-bit in LDS, J and H rows read from the batch's tables in global memory at the start of
-// each run, and per region (Region below) a mask of where the rows continue (bit j: row j is row j - 1 plus one)
+// matrix): input / F rows 32-bit in LDS, J and H rows in registers (lane j holds tile matrix j's, read across lanes
+// by ds_bpermute: every call of jr / hr must be made by the whole wavefront), and per region (Region below) a mask
+// of where the rows continue (bit j: row j is row j - 1 plus one).  base: the tile matrix of index 0 (shift_rows).
 struct ListRows32 {
   static constexpr bool RUN = false, MASKS = true, SCATTER = false;
-  const int32_t* mr;              // LDS
-  const int64_t* jrr;             // global
-  const int64_t* hrr;
+  const int32_t* mr;              // LDS, from tile matrix base
+  int64_t jl, hl;                 // this lane's tile matrix's J / H row
+  int base;
   uint64_t cont[3];
   MF_DEV int64_t m(int j) const { return (int64_t)mr[j]; }
-  MF_DEV int64_t jr(int j) const { return jrr[j]; }
-  MF_DEV int64_t hr(int j) const { return hrr[j]; }
+  MF_DEV int64_t jr(int j) const { return __shfl(jl, (base + j) & (WAVE - 1)); }
+  MF_DEV int64_t hr(int j) const { return __shfl(hl, (base + j) & (WAVE - 1)); }
 };
 
 // rows given by a permutation (the length-sorted general-matrix path): tile matrix j is row p[j] of the input and
@@ -80,7 +81,7 @@ MF_DEV const T* opaque(const T* p) {
 // the rows of tile matrices p0, p0 + 1, ... (a pass over part of the tile)
 MF_DEV RunRows shift_rows(const RunRows& r, int p0) { return RunRows{r.base + p0}; }
 MF_DEV ListRows32 shift_rows(const ListRows32& r, int p0) {
-  return ListRows32{r.mr + p0, r.jrr + p0, r.hrr + p0, {r.cont[0] >> p0, r.cont[1] >> p0, r.cont[2] >> p0}};
+  return ListRows32{r.mr + p0, r.jl, r.hl, r.base + p0, {r.cont[0] >> p0, r.cont[1] >> p0, r.cont[2] >> p0}};
 }
 MF_DEV ListRows shift_rows(const ListRows& r, int p0) { return ListRows{r.mr + p0, r.jrr + p0, r.hrr + p0}; }
 MF_DEV PermRows shift_rows(const PermRows& r, int p0) {
