@@ -14,11 +14,12 @@ Key "mixed_vs_fp64": the mixed form against the FP64 form on smf log and smf pow
 uniform in [-1, 1] (order 2, n = 2^24, the headline distribution, seed 1) plus gmf log order 1 (n = 2^22, the general
 mix, seed 1): time per step of each form, the speed-up, and the mixed form's error against the FP64 form per output.
 Key "hetero_batch": a heterogeneous batch of 2^20 finite elements (U{1,4,8,10,27} points; smf log o2 / smf pow o1
-with per-matrix exponents / gmf exp o1 / gmf log o2) evaluated by one kernel, 20 Newton iterations per hipGraph
-replay, launch strategy tuned, FP64 and mixed (hetero_batch's docstring).
+with per-matrix exponents / gmf exp o1 / gmf log o2) evaluated through the plan's tuned launch strategy, 20 Newton
+iterations per hipGraph replay (each perturbing A inside the evaluation's kernels), FP64 and mixed (hetero_batch's
+docstring).
 
     python bench.py [--steps K] [--warmup W] [--n N] [--gmf-n N] [--gpus 1] [--no-cpu] [--only-headline]
-                    [--only-order0]
+                    [--only-order0] [--only-batch]
                     [--mixed-n N] [--mixed-gmf-n N] [--batch-elements E] [--batch-iters I]
 
 `value` = matrices/s of the headline step (4 n matrices per step).  `roofline`: the dominant kernel (the smf order-2
@@ -333,6 +334,7 @@ def main():
     ap.add_argument("--batch-iters", type=int, default=20)
     ap.add_argument("--no-tune", action="store_true")
     ap.add_argument("--only-gmf", action="store_true", help="only the general-matrix legs (for the counter runs)")
+    ap.add_argument("--only-batch", action="store_true", help="only the heterogeneous batch leg")
     ap.add_argument("--gmf-strategy", type=int, choices=(0, 1, 2), default=None,
                     help="every function, order and precision of the sorted entry points on this strategy")
     args = ap.parse_args()
@@ -525,6 +527,9 @@ def main():
                     assert lib.gmf_hip_set_sort_strategy(k, o, x, args.gmf_strategy) == 0
     if args.only_order0:
         print(json.dumps({"order0": order0_legs()}), flush=True)
+        return
+    if args.only_batch:
+        print(json.dumps({"hetero_batch": hetero_batch(torch, W, dev, stream, args)}), flush=True)
         return
     if args.only_gmf:
         work = torch.empty(lib.gmf_hip_sort_workspace(ng), dtype=torch.uint8, device=dev)

@@ -196,8 +196,9 @@ int mf_hip_plan_tune(mf_hip_plan *plan, const double *A, const double *alpha, vo
 int mf_hip_perturb(const mf_hip_plan *plan, double *A, double rel_sigma, uint64_t seed, int iteration, void *stream);
 
 /* A sequence of Newton iterations captured in one hipGraph: A <- A0 (device copy), then `iterations` times
- * { mf_hip_perturb(iteration i = 1..iterations); mf_hip_batch_eval(_mixed) } with the plan's strategy at creation.
- * F / J / H hold the last iteration's results.  Replays are independent (each starts from A0).  Creation captures and
+ * { mf_hip_perturb(iteration i = 1..iterations); mf_hip_batch_eval(_mixed) } with the plan's strategy at creation --
+ * the perturbation applied by the evaluation's own kernels as they read A (one pass over A per iteration; A and the
+ * results bitwise those of the two calls).  F / J / H hold the last iteration's results.  Replays are independent (each starts from A0).  Creation captures and
  * instantiates (synchronous); launch is asynchronous on `stream`.  The sequence keeps using the plan's tables and the
  * buffers given here: destroy it before the plan (mf_hip_plan_destroy refuses until then) and keep the buffers. */
 int mf_hip_sequence_create(const mf_hip_plan *plan, const double *A0, double *A, const double *alpha, void *F, void *J,

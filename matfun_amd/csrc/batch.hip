@@ -102,8 +102,9 @@ double tile_cost(int c) {
   return cls_fam(c) == 0 ? (o == 2 ? 1.0 : (o == 1 ? 0.15 : 0.05)) : (o == 2 ? 1.05 : (o == 1 ? 0.45 : 0.3));
 }
 
+// Aw: with the perturbation pt applied by the evaluation (A itself, writable), else null
 BatchArgs args_of(const mf_hip_plan* p, const BatchTile* tiles, int64_t t0, int64_t t1, const double* A,
-                  const double* alpha) {
+                  const double* alpha, double* Aw = nullptr, const BatchPerturb* pt = nullptr) {
   BatchArgs a;
   a.tiles = tiles;
   a.t0 = t0;
@@ -113,6 +114,9 @@ BatchArgs args_of(const mf_hip_plan* p, const BatchTile* tiles, int64_t t0, int6
   a.rh = p->d_rh;
   a.A = A;
   a.alv = alpha ? alpha : p->d_alpha;
+  a.perturb = Aw && pt ? 1 : 0;
+  a.Aw = Aw;
+  a.pt = pt ? *pt : BatchPerturb{0.0, 0, 0};
   return a;
 }
 
@@ -140,10 +144,12 @@ size_t staged_work_bytes(const mf_hip_plan* p) {
 // mode 0: the kernel over all classes; mode 1: one kernel per class; mode 3: as mode 1, the gmf classes of order >= 1
 // other than exp through the staged evaluation (gmf_staged_list, then their tiles finished from the staged data).
 // per_cu > 0: persistent, per_cu workgroups per compute unit; 0: one workgroup per tile.  order (mode 0): 0
-// class-major tiles, 1 interleaved.
+// class-major tiles, 1 interleaved.  pt (with Aw = A): each matrix perturbed first by the kernel that evaluates it
+// (or by the staged nodes pass), its perturbed inputs written back over A -- mf_hip_perturb then the evaluation,
+// bitwise, in one pass over A.
 template <class OT>
 int batch_eval(const mf_hip_plan* p, const double* A, const double* alpha, OT* F, OT* J, OT* H, hipStream_t st,
-               int mode, int per_cu, int order) {
+               int mode, int per_cu, int order, const BatchPerturb* pt = nullptr, double* Aw = nullptr) {
   if (!p || (p->N > 0 && (!A || !F)) || (p->needJ && !J) || (p->needH && !H)) return -1;
   if (p->ntiles == 0) return 0;
   if (mode == 3) {
@@ -151,14 +157,17 @@ int batch_eval(const mf_hip_plan* p, const double* A, const double* alpha, OT* F
     for (int c = 0; c < NCLS; c++) {
       const int64_t n = p->cls_t1[c] - p->cls_t0[c];
       if (n == 0) continue;
-      const BatchArgs a = args_of(p, p->d_sorted, p->cls_t0[c], p->cls_t1[c], A, alpha);
+      const BatchArgs a = args_of(p, p->d_sorted, p->cls_t0[c], p->cls_t1[c], A, alpha, Aw, pt);
       if (batch_gmf_stageable(c)) {
         const int slot = cls_slot(c), ord = cls_order(c);
         const int64_t off0 = p->tiles[(size_t)p->cls_t0[c]].off;
         const GmfCoef& bt = slot == SLOT_POW ? p->tab.recips : p->tab.gmf[slot];
         const GmfStaged sg = gmf_staged_list(st, slot == 1 ? LOG : POW, ord, slot == SLOT_POW, cls_matrices(p, c),
-                                             p->d_rm + off0, A, slot == 2 ? 0.5 : 1.0, a.alv, bt, p->d_work);
-        batch_launch_gmf_staged(c, st, (unsigned)n, a, off0, sg, F, J, H);
+                                             p->d_rm + off0, A, slot == 2 ? 0.5 : 1.0, a.alv, bt, p->d_work,
+                                             a.perturb ? Aw : nullptr, a.pt.sigma, a.pt.seed, a.pt.iteration);
+        BatchArgs af = a;                       // (the finishing pass reads the inputs the nodes pass perturbed)
+        af.perturb = 0;
+        batch_launch_gmf_staged(c, st, (unsigned)n, af, off0, sg, F, J, H);
       } else if (cls_fam(c) == 0) {
         batch_launch_smf(c, st, grid_for(p, n, per_cu), per_cu > 0, a, F, J, H);
       } else {
@@ -168,7 +177,7 @@ int batch_eval(const mf_hip_plan* p, const double* A, const double* alpha, OT* F
     return hipGetLastError() == hipSuccess ? 0 : -2;
   }
   if (mode == 0) {
-    const BatchArgs a = args_of(p, order == 1 ? p->d_mixed : p->d_sorted, 0, p->ntiles, A, alpha);
+    const BatchArgs a = args_of(p, order == 1 ? p->d_mixed : p->d_sorted, 0, p->ntiles, A, alpha, Aw, pt);
     batch_launch_fused(st, grid_for(p, p->ntiles, per_cu), per_cu > 0, a, F, J, H);
     return hipGetLastError() == hipSuccess ? 0 : -2;
   }
@@ -184,7 +193,7 @@ int batch_eval(const mf_hip_plan* p, const double* A, const double* alpha, OT* F
       used[k] = true;
       s = p->side[k];
     }
-    const BatchArgs a = args_of(p, p->d_sorted, p->cls_t0[c], p->cls_t1[c], A, alpha);
+    const BatchArgs a = args_of(p, p->d_sorted, p->cls_t0[c], p->cls_t1[c], A, alpha, Aw, pt);
     if (cls_fam(c) == 0) batch_launch_smf(c, s, grid_for(p, n, per_cu), per_cu > 0, a, F, J, H);
     else batch_launch_gmf(c, s, grid_for(p, n, per_cu), per_cu > 0, a, F, J, H);
   }
@@ -458,10 +467,12 @@ extern "C" int mf_hip_sequence_create(const mf_hip_plan* p, const double* A0, do
       hipMemcpyAsync(A, A0, (size_t)p->N * 9 * sizeof(double), hipMemcpyDeviceToDevice, cs) != hipSuccess)
     rc = -2;
   for (int it = 1; rc == 0 && it <= iterations; it++) {
-    rc = mf_hip_perturb(p, A, rel_sigma, seed, it, cs);
-    if (rc == 0)
-      rc = mixed ? batch_eval(p, A, alpha, (float*)F, (float*)J, (float*)H, cs, p->mode, p->per_cu, p->order)
-                 : batch_eval(p, A, alpha, (double*)F, (double*)J, (double*)H, cs, p->mode, p->per_cu, p->order);
+    // (the perturbation of iteration it applied by the evaluation's kernels: mf_hip_perturb then the evaluation,
+    // bitwise, with one pass over A)
+    const BatchPerturb pt{rel_sigma, seed, (uint32_t)it};
+    rc = mixed ? batch_eval(p, A, alpha, (float*)F, (float*)J, (float*)H, cs, p->mode, p->per_cu, p->order, &pt, A)
+               : batch_eval(p, A, alpha, (double*)F, (double*)J, (double*)H, cs, p->mode, p->per_cu, p->order, &pt,
+                            A);
   }
   hipGraph_t g = nullptr;
   const hipError_t ec = hipStreamEndCapture(cs, &g);

@@ -12,6 +12,7 @@
 // the persistent loop keeps no per-tile state live across tiles, so its register use stays that of one tile.
 #pragma once
 #include "gmf_tile.hpp"
+#include "philox.hpp"
 #include "smf_tile.hpp"
 
 namespace mf {
@@ -35,6 +36,14 @@ struct BatchTables {     // the same for every plan
   GmfCoef recips;                 // 1 / j (per-matrix exponents)
 };
 
+// the Newton perturbation applied by the evaluation itself: each matrix's inputs multiplied by perturb_factors
+// (philox.hpp) of its row before it is evaluated, and written back over A (mf_hip_perturb's result, bitwise)
+struct BatchPerturb {
+  double sigma;
+  uint64_t seed;
+  uint32_t iteration;
+};
+
 struct BatchArgs {
   const BatchTile* tiles;
   int64_t t0, t1;                 // tile range of this launch
@@ -43,6 +52,23 @@ struct BatchArgs {
   const int64_t* rh;
   const double* A;
   const double* alv;              // per-matrix exponents (the caller's, or the plan's group exponents)
+  int perturb;                    // 1: the inputs perturbed first (pt), written back through Aw (= A)
+  double* Aw;
+  BatchPerturb pt;
+};
+
+// the tile bodies' input transform (pre_inputs in tile_rows.hpp) for BatchArgs::perturb
+struct PerturbPre {
+  double* x;
+  BatchPerturb pt;
+  bool on, sym;
+  MF_DEV bool active() const { return on; }
+  MF_DEV void apply(double* a, int64_t m) const {
+    double f[9];
+    perturb_factors(pt.seed, pt.iteration, (uint64_t)m, sym, pt.sigma, f);
+#pragma unroll
+    for (int k = 0; k < 9; k++) a[k] *= f[k];
+  }
 };
 
 // ---- host side: each kernel file holds its own copy of the tables in constant memory (code objects do not share
@@ -78,16 +104,18 @@ MF_DEV void run_tile(const BatchArgs& a, const BatchTile& tl, const R& rows, OT*
                      unsigned char* lds, const short* lutJ, const short* lutH) {
   constexpr int FAM = cls_fam(CLS), SLOT = cls_slot(CLS), ORDER = cls_order(CLS);
   const int lane = opaque((int)threadIdx.x);
+  const PerturbPre pre{a.Aw, a.pt, a.perturb != 0, FAM == 0};
   if constexpr (FAM == 0) {
     constexpr int KIND = SLOT == 0 ? EXP : (SLOT == 1 ? LOG : (SLOT == 2 ? SQRT : POW));
     const double* c2 = opaque(g_tab.smf_c2[SLOT < 3 ? SLOT : 0]);
-    smf_tile<KIND, true, ORDER, OT, 1, SLOT == SLOT_POW>(rows, tl.nm, a.A, F, J, H, SLOT == 2 ? 0.5 : 0.0,
-                                                              a.alv, c2, lds, lutJ, lutH, lane);
+    smf_tile<KIND, true, ORDER, OT, 1, SLOT == SLOT_POW, R, HPP_DEFAULT, false, false, PerturbPre>(
+        rows, tl.nm, a.A, F, J, H, SLOT == 2 ? 0.5 : 0.0, a.alv, c2, lds, lutJ, lutH, lane, pre);
   } else {
     constexpr int KIND = SLOT == 0 ? EXP : (SLOT == 1 ? LOG : POW);
     const double* bt = opaque(SLOT == SLOT_POW ? g_tab.recips.b : g_tab.gmf[SLOT].b);
-    gmf_tile<KIND, ORDER, OT, SLOT == SLOT_POW>(rows, tl.nm, a.A, F, J, H, SLOT == 2 ? 0.5 : 0.0, a.alv, bt, lds,
-                                                 lane);
+    gmf_tile<KIND, ORDER, OT, SLOT == SLOT_POW, R, 1, false, PerturbPre>(
+        rows, tl.nm, a.A, F, J, H, SLOT == 2 ? 0.5 : 0.0, a.alv, bt, lds, lane, nullptr, 0,
+        GmfStaged{nullptr, nullptr, nullptr, nullptr, 0, -1}, pre);
   }
 }
 

@@ -21,6 +21,7 @@
 
 #include "gmf_sorted.hpp"
 #include "gmf_tile.hpp"
+#include "philox.hpp"
 #include "smf_launch.hpp"
 
 namespace mf {
@@ -51,18 +52,30 @@ __global__ void __launch_bounds__(KLEN_THREADS)
 }
 
 // the series inputs and the key of matrices k = 0 .. n-1 (one lane per matrix; the first pass of the staged
-// evaluation): matrix k is row rows[k] of A (and of alv) when rows is given, else row k; outputs at k
+// evaluation): matrix k is row rows[k] of A (and of alv) when rows is given, else row k; outputs at k.  Aw (= A,
+// hence A without __restrict__): the inputs first multiplied by the matrix's Newton perturbation factors and written
+// back
 template <int KIND, int ORDER, bool PA>
 __global__ void __launch_bounds__(KLEN_THREADS)
-    gmf_prep_kernel(int64_t n, const int64_t* __restrict__ rows, const double* __restrict__ A, double al,
+    gmf_prep_kernel(int64_t n, const int64_t* __restrict__ rows, const double* A, double al,
                     const double* __restrict__ alv, GmfCoef bt, uint8_t* __restrict__ key, GmfSeriesIn* __restrict__ sin,
-                    GmfNodesRe* __restrict__ nre, double* __restrict__ nim) {
+                    GmfNodesRe* __restrict__ nre, double* __restrict__ nim, double* Aw, double sigma, uint64_t seed,
+                    uint32_t iteration) {
   const int64_t k = (int64_t)blockIdx.x * KLEN_THREADS + threadIdx.x;
   if (k >= n) return;
   const int64_t i = rows ? rows[k] : k;
   double a[9];
 #pragma unroll
   for (int k = 0; k < 9; k++) a[k] = A[9 * i + k];
+  if (Aw) {
+    double f[9];
+    perturb_factors(seed, iteration, (uint64_t)i, false, sigma, f);
+#pragma unroll
+    for (int k = 0; k < 9; k++) {
+      a[k] *= f[k];
+      Aw[9 * i + k] = a[k];
+    }
+  }
   if (PA) {
     al = alv[i];
     al = fabs(al) <= 1.7976931348623157e308 ? al : 1.0;
@@ -279,7 +292,8 @@ size_t staged_bytes(int64_t m, int order) {
 // row rows[k] of A (or row k): the staged data, indexed by k, in the workspace (staged_bytes(m, ORDER))
 template <int KIND, int ORDER, bool PA>
 GmfStaged staged_front(hipStream_t st, int64_t m, const int64_t* rows, const double* A, double al, const double* alv,
-                       const GmfCoef& bt, void* work) {
+                       const GmfCoef& bt, void* work, double* Aw = nullptr, double sigma = 0.0, uint64_t seed = 0,
+                       uint32_t iteration = 0) {
   const SortSpace s = carve(work, m);
   unsigned char* p = reinterpret_cast<unsigned char*>(s.off) + align256(4 * (size_t)KSORT_MAXCH * KBINS);
   GmfSeriesIn* sin = reinterpret_cast<GmfSeriesIn*>(p);
@@ -297,7 +311,7 @@ GmfStaged staged_front(hipStream_t st, int64_t m, const int64_t* rows, const dou
   const int nch = (int)((m + chunk - 1) / chunk);
   const unsigned gl = (unsigned)((m + KLEN_THREADS - 1) / KLEN_THREADS);
   hipLaunchKernelGGL((gmf_prep_kernel<KIND, ORDER, PA>), dim3(gl), dim3(KLEN_THREADS), 0, st, m, rows, A, al, alv, bt,
-                     s.key, sin, nre, nim);
+                     s.key, sin, nre, nim, Aw, sigma, seed, iteration);
   hipLaunchKernelGGL(key_hist_kernel, dim3(nch), dim3(SORT_THREADS), 0, st, m, chunk, s.key, s.hist);
   hipLaunchKernelGGL(key_scan_kernel, dim3(1), dim3(SORT_THREADS), 0, st, nch, s.hist, s.off);
   hipLaunchKernelGGL(key_scatter_kernel, dim3(nch), dim3(SORT_THREADS), 0, st, m, chunk, s.key, s.off, s.perm);
@@ -377,11 +391,12 @@ void plain_launch(hipStream_t st, int slot, int kind, int64_t n, const double* A
 size_t gmf_staged_list_bytes(int64_t m, int order) { return m > 0 ? staged_bytes(m, order) : 0; }
 
 GmfStaged gmf_staged_list(hipStream_t st, int kind, int order, bool pa, int64_t m, const int64_t* rows,
-                          const double* A, double al, const double* alv, const GmfCoef& bt, void* work) {
-#define MF_FRONT(K, P)                                                                       \
-  if (order == 2) return staged_front<K, 2, P>(st, m, rows, A, al, alv, bt, work);          \
-  if (order == 1) return staged_front<K, 1, P>(st, m, rows, A, al, alv, bt, work);          \
-  return staged_front<K, 0, P>(st, m, rows, A, al, alv, bt, work);
+                          const double* A, double al, const double* alv, const GmfCoef& bt, void* work, double* Aw,
+                          double sigma, uint64_t seed, uint32_t iteration) {
+#define MF_FRONT(K, P)                                                                                      \
+  if (order == 2) return staged_front<K, 2, P>(st, m, rows, A, al, alv, bt, work, Aw, sigma, seed, iteration); \
+  if (order == 1) return staged_front<K, 1, P>(st, m, rows, A, al, alv, bt, work, Aw, sigma, seed, iteration); \
+  return staged_front<K, 0, P>(st, m, rows, A, al, alv, bt, work, Aw, sigma, seed, iteration);
   if (pa) { MF_FRONT(POW, true) }
   if (kind == EXP) { MF_FRONT(EXP, false) }
   if (kind == LOG) { MF_FRONT(LOG, false) }

@@ -34,11 +34,14 @@ int64_t gmf_sort_capacity(size_t bytes);
 // the passes of the staged evaluation before its finishing pass for a list of m matrices (a heterogeneous batch's
 // class): matrix k of the list is row rows[k] of A (and of alv, per-matrix exponents when pa); kind normalised as
 // gmf_hip_eval (sqrt as pow 1/2 with its table bt).  Launched on st into the workspace work
-// (gmf_staged_list_bytes(m, order) bytes); returns where the staged data lie, indexed by k.
+// (gmf_staged_list_bytes(m, order) bytes); returns where the staged data lie, indexed by k.  With Aw (= A) the nodes
+// pass first multiplies each matrix's inputs by its Newton perturbation factors (perturb_factors in philox.hpp:
+// sigma, seed, iteration; general, not symmetric) and writes them back over A.
 struct GmfStaged;
 size_t gmf_staged_list_bytes(int64_t m, int order);
 GmfStaged gmf_staged_list(hipStream_t st, int kind, int order, bool pa, int64_t m, const int64_t* rows,
-                          const double* A, double al, const double* alv, const GmfCoef& bt, void* work);
+                          const double* A, double al, const double* alv, const GmfCoef& bt, void* work,
+                          double* Aw = nullptr, double sigma = 0.0, uint64_t seed = 0, uint32_t iteration = 0);
 
 // the batch through the slot's strategy, in parts of what the workspace holds (the staged evaluation falls back to
 // the permuted one when the workspace holds fewer than 2^15 matrices of it); kind normalised as gmf_hip_eval; alv:

@@ -114,12 +114,13 @@ struct GmfStaged {
   int64_t off, pos0;
 };
 
-// STAGED: each lane finishes its matrix from staged nodes and sums (gmf_finish) instead of running gmf_lane
-template <int KIND, int ORDER, class OT, bool PA, class R, int NW = 1, bool STAGED = false>
+// STAGED: each lane finishes its matrix from staged nodes and sums (gmf_finish) instead of running gmf_lane.
+// pre: a transform of the inputs first (pre_inputs in tile_rows.hpp)
+template <int KIND, int ORDER, class OT, bool PA, class R, int NW = 1, bool STAGED = false, class Pre = NoPre>
 MF_DEV void gmf_tile(const R& rows, int nm, const double* __restrict__ A, OT* __restrict__ F, OT* __restrict__ J,
                      OT* __restrict__ H, double al, const double* __restrict__ alv, const double* __restrict__ bt,
                      unsigned char* lds, int lane, unsigned char* pool = nullptr, int tid = 0,
-                     GmfStaged staged = GmfStaged{nullptr, nullptr, nullptr, nullptr, 0, -1}) {
+                     GmfStaged staged = GmfStaged{nullptr, nullptr, nullptr, nullptr, 0, -1}, const Pre& pre = Pre()) {
   using L = GmfLds<ORDER, OT>;
   constexpr int VEC = 16 / (int)sizeof(OT);             // (elements per 16-byte store)
   double* xin = reinterpret_cast<double*>(lds);
@@ -131,6 +132,7 @@ MF_DEV void gmf_tile(const R& rows, int nm, const double* __restrict__ A, OT* __
   double a[9];
 #pragma unroll
   for (int k = 0; k < 9; k++) a[k] = lane < nm ? xin[9 * lane + k] : (k % 4 == 0 ? 1.0 : 0.0);
+  pre_inputs(pre, a, xin, rows, nm, lane);
   bool alok = true;                                  // (PA: a non-finite exponent computes with 1 and stores NaN)
   if (PA) {
     al = lane < nm ? alv[rows.m(lane < nm ? lane : 0)] : 1.0;

@@ -68,11 +68,12 @@ MF_DEV void smf_luts(short* lutJ, short* lutH, int lane) {
 // one tile of nm <= 64 matrices; lds: SmfLds<MAT, ORDER, OT, HPP, SEP>::BYTES bytes, 16-byte aligned; the LUTs
 // filled (smf_luts) and visible to the wave; lane = threadIdx.x; VEC: elements per lane and store of J and H
 // (copy_out, copy_out_lut).  Ends with the tile's outputs written and a barrier (the LDS may be reused).
+// pre: a transform of the inputs first (pre_inputs in tile_rows.hpp; matrix layout only)
 template <int KIND, bool MAT, int ORDER, class OT, int VEC, bool PA, class R, int HPP = HPP_DEFAULT,
-          bool SEP = false, bool PACK = false>
+          bool SEP = false, bool PACK = false, class Pre = NoPre>
 MF_DEV void smf_tile(const R& rows, int nm, const double* __restrict__ x, OT* __restrict__ F, OT* __restrict__ J,
                      OT* __restrict__ H, double al, const double* __restrict__ alv, const double* __restrict__ c2,
-                     unsigned char* lds, const short* lutJ, const short* lutH, int lane) {
+                     unsigned char* lds, const short* lutJ, const short* lutH, int lane, const Pre& pre = Pre()) {
   using L = SmfLds<MAT, ORDER, OT, HPP, SEP>;
   static_assert(!(PACK && MAT), "packed outputs take Voigt input");
   constexpr int NIN = L::NIN, NF = MAT ? 9 : 6, NJ = PACK ? 21 : NF * NF, NH = PACK ? 126 : NF * NF * NF;
@@ -89,6 +90,7 @@ MF_DEV void smf_tile(const R& rows, int nm, const double* __restrict__ x, OT* __
     double a[9];
 #pragma unroll
     for (int k = 0; k < 9; k++) a[k] = lane < nm ? xin[9 * lane + k] : (k % 4 == 0 ? 1.0 : 0.0);
+    pre_inputs(pre, a, xin, rows, nm, lane);
     matrix_to_voigt(a, v);
   } else {
 #pragma unroll

@@ -343,4 +343,45 @@ MF_DEV void load_in(double* __restrict__ xin, const double* __restrict__ x, cons
   }
 }
 
+// the reverse of load_in: the tile's inputs from LDS back to x, coalesced
+template <int NIN, class R>
+MF_DEV void store_in(double* __restrict__ x, const double* __restrict__ xin, const R& rows, int nm, int lane) {
+  if (R::RUN) {
+    double* blk = x + rows.m(0) * NIN;
+    for (int t = lane; t < NIN * nm; t += WAVE) blk[t] = xin[t];
+  } else {
+    int j = lane / NIN, e = lane - j * NIN;
+    for (int t = lane; t < NIN * nm; t += WAVE) {
+      x[rows.m(j) * NIN + e] = xin[t];
+      e += WAVE;
+      while (e >= NIN) { e -= NIN; j++; }
+    }
+  }
+}
+
+// A transform of each tile matrix's inputs before it is evaluated, the transformed inputs also written back over
+// the input rows (the heterogeneous batch's Newton perturbation).  Pre: active() (uniform), apply(a, row) on one
+// lane's 9 inputs, and x, where the inputs are written back.  NoPre: none (the evaluation entry points).
+struct NoPre {
+  double* x = nullptr;
+  MF_DEV bool active() const { return false; }
+  MF_DEV void apply(double*, int64_t) const {}
+};
+
+// pre applied to the tile's matrices (lane j: a = its 9 inputs, already read from xin, which held the tile's
+// inputs; its row rows.m(j)), the results written back to pre.x through xin; the wave's barriers included
+template <class Pre, class R>
+MF_DEV void pre_inputs(const Pre& pre, double* a, double* xin, const R& rows, int nm, int lane) {
+  if (!pre.active()) return;
+  wave_sync();                                     // (every lane has read its inputs from xin)
+  if (lane < nm) {
+    pre.apply(a, rows.m(lane));
+#pragma unroll
+    for (int k = 0; k < 9; k++) xin[9 * lane + k] = a[k];
+  }
+  wave_sync();
+  store_in<9>(pre.x, xin, rows, nm, lane);
+  wave_sync();                                     // (xin free again)
+}
+
 }  // namespace mf

@@ -203,10 +203,12 @@ def test_perturbation_matches_cpu_build_and_keeps_symmetry():
     b.close()
 
 
-@pytest.mark.parametrize("mixed,mode", [(False, 0), (True, 0), (False, 2), (False, 3), (True, 3)])
+@pytest.mark.parametrize("mixed,mode", [(False, 0), (True, 0), (False, 1), (False, 2), (False, 3), (True, 3)])
 def test_graph_sequence_equals_eager_steps(mixed, mode):
-    """mode 2 forks the class kernels onto the plan's streams: the capture follows the fork and the join; mode 3 runs
-    the staged classes' passes through the plan's workspace"""
+    """the graph's iterations perturb A inside the evaluation's kernels (the staged classes' nodes pass in mode 3),
+    the eager steps call the perturbation kernel (mf_hip_perturb) and then the evaluation: A and the outputs bitwise
+    equal.  mode 2 forks the class kernels onto the plan's streams: the capture follows the fork and the join; mode 3
+    runs the staged classes' passes through the plan's workspace"""
     import torch
     import matfun_amd
     gs = groups_all_classes(610)
