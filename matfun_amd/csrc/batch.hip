@@ -117,6 +117,9 @@ BatchArgs args_of(const mf_hip_plan* p, const BatchTile* tiles, int64_t t0, int6
   a.perturb = Aw && pt ? 1 : 0;
   a.Aw = Aw;
   a.pt = pt ? *pt : BatchPerturb{0.0, 0, 0};
+  // (read by the class launches, whose t0 .. t1 are one class's tiles in the class-major order of p->tiles)
+  a.coff = t1 > t0 ? p->tiles[(size_t)t0].off : 0;
+  a.cn = t1 > t0 ? (t1 - t0 - 1) * WAVE + p->tiles[(size_t)(t1 - 1)].nm : 0;
   return a;
 }
 

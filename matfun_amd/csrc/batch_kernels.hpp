@@ -55,7 +55,16 @@ struct BatchArgs {
   int perturb;                    // 1: the inputs perturbed first (pt), written back through Aw (= A)
   double* Aw;
   BatchPerturb pt;
+  // a launch over one class's tiles (full but the last): they list the class's cn matrices from table offset coff
+  int64_t coff, cn;
 };
+
+// tile t of a launch over one class's tiles, from its place (no descriptor load)
+template <int CLS>
+MF_DEV BatchTile class_tile(const BatchArgs& a, int64_t t) {
+  const int64_t k = (t - a.t0) * WAVE, left = a.cn - k;
+  return BatchTile{CLS, (int32_t)(left < WAVE ? left : WAVE), a.coff + k};
+}
 
 // the tile bodies' input transform (pre_inputs in tile_rows.hpp) for BatchArgs::perturb
 struct PerturbPre {
@@ -193,7 +202,7 @@ batch_class_kernel(BatchArgs a, OT* __restrict__ F, OT* __restrict__ J, OT* __re
   const int lane = threadIdx.x;
   if (FAM == 0) smf_luts<true, ORDER, OT>(lutJ, lutH, lane);
   for (int64_t t = a.t0 + blockIdx.x; t < a.t1; t += gridDim.x) {
-    const BatchTile tl = a.tiles[t];
+    const BatchTile tl = class_tile<CLS>(a, t);
     run_tile<CLS, OT>(a, tl, stage_rows32(a, tl, rows32, lane), F, J, H, lds, lutJ, lutH);
     if (!LOOP) break;
   }
@@ -212,7 +221,7 @@ batch_staged_kernel(BatchArgs a, int64_t off0, GmfStaged sg, OT* __restrict__ F,
   const int lane = threadIdx.x;
   const int64_t t = a.t0 + blockIdx.x;
   if (t >= a.t1) return;
-  const BatchTile tl = a.tiles[t];
+  const BatchTile tl = class_tile<CLS>(a, t);
   const double* bt = SLOT == SLOT_POW ? g_tab.recips.b : g_tab.gmf[SLOT].b;
   const GmfStaged s{sg.nre, sg.nim, sg.sre, sg.sim, 0, tl.off - off0};
   gmf_tile<KIND, ORDER, OT, SLOT == SLOT_POW, ListRows32, 1, true>(stage_rows32(a, tl, rows32, lane), tl.nm, a.A, F, J,
