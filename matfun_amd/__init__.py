@@ -483,8 +483,13 @@ class Batch:
         return F, J, H
 
     def set_strategy(self, mode, per_cu, order=1):
-        if self._lib.mf_hip_plan_set_strategy(self._h, int(mode), int(per_cu), int(order)) != 0:
-            raise ValueError("invalid strategy")
+        """mf_hip_plan_set_strategy: mode 0 fused, 1 split, 2 concurrent split, 3 staged split (allocates the plan's
+        workspace for the staged classes on first use)"""
+        r = self._lib.mf_hip_plan_set_strategy(self._h, int(mode), int(per_cu), int(order))
+        if r == -2:
+            raise MemoryError("mf_hip_plan_set_strategy: the staged classes' workspace could not be allocated")
+        if r != 0:
+            raise ValueError(f"invalid strategy (mode {mode}, per_cu {per_cu}, order {order})")
 
     def tune(self, A, alpha=None, out=None, precision="fp64"):
         import torch

@@ -261,9 +261,16 @@ def test_invalid_plans_and_calls():
     assert lib.mf_hip_batch_eval(b._h, A.data_ptr(), None, F.data_ptr(), J.data_ptr(), None, None) == -1
     assert lib.mf_hip_batch_eval(b._h, A.data_ptr(), None, F.data_ptr(), None, H.data_ptr(), None) == -1
     assert lib.mf_hip_batch_eval(None, A.data_ptr(), None, F.data_ptr(), J.data_ptr(), H.data_ptr(), None) == -1
+    for bad in ((4, 0, 0), (-1, 0, 0), (1, 65, 0), (0, 0, 2)):
+        with pytest.raises(ValueError):
+            b.set_strategy(*bad)
+    b.set_strategy(3, 0, 0)                          # (no staged class in this plan: no workspace needed)
+    assert lib.mf_hip_batch_eval(b._h, A.data_ptr(), None, F.data_ptr(), J.data_ptr(), H.data_ptr(), None) == 0
+    torch.cuda.synchronize()
     empty = matfun_amd.Batch([])
     assert empty.n == 0
     empty.eval(None, out=(torch.empty(0), torch.empty(0), torch.empty(0)))
+    empty.set_strategy(3, 0, 0)
     b.close()
     empty.close()
 
