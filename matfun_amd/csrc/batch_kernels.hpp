@@ -139,13 +139,15 @@ MF_DEV void stage_rows(const BatchArgs& a, const BatchTile& tl, int64_t* rows, i
 
 // ListRows32 of a tile: its input rows into LDS (32-bit: a plan holds fewer than 2^31 matrices), its J and H rows
 // in registers, the run masks of the three regions by ballot
+// (ORDER: the class's order; the J / H rows of a class that writes no J / H are not read)
+template <int ORDER>
 MF_DEV ListRows32 stage_rows32(const BatchArgs& a, const BatchTile& tl, int32_t* rows, int lane) {
   const bool in = lane < tl.nm;
   int64_t m = 0, jr = 0, hr = 0;
   if (in) {
     m = a.rm[tl.off + lane];
-    jr = a.rj[tl.off + lane];
-    hr = a.rh[tl.off + lane];
+    if (ORDER >= 1) jr = a.rj[tl.off + lane];
+    if (ORDER >= 2) hr = a.rh[tl.off + lane];
     rows[lane] = (int32_t)m;
   }
   const int64_t pm = __shfl_up(m, 1), pj = __shfl_up(jr, 1), ph = __shfl_up(hr, 1);
@@ -203,7 +205,7 @@ batch_class_kernel(BatchArgs a, OT* __restrict__ F, OT* __restrict__ J, OT* __re
   if (FAM == 0) smf_luts<true, ORDER, OT>(lutJ, lutH, lane);
   for (int64_t t = a.t0 + blockIdx.x; t < a.t1; t += gridDim.x) {
     const BatchTile tl = class_tile<CLS>(a, t);
-    run_tile<CLS, OT>(a, tl, stage_rows32(a, tl, rows32, lane), F, J, H, lds, lutJ, lutH);
+    run_tile<CLS, OT>(a, tl, stage_rows32<ORDER>(a, tl, rows32, lane), F, J, H, lds, lutJ, lutH);
     if (!LOOP) break;
   }
 }
@@ -224,9 +226,9 @@ batch_staged_kernel(BatchArgs a, int64_t off0, GmfStaged sg, OT* __restrict__ F,
   const BatchTile tl = class_tile<CLS>(a, t);
   const double* bt = SLOT == SLOT_POW ? g_tab.recips.b : g_tab.gmf[SLOT].b;
   const GmfStaged s{sg.nre, sg.nim, sg.sre, sg.sim, 0, tl.off - off0};
-  gmf_tile<KIND, ORDER, OT, SLOT == SLOT_POW, ListRows32, 1, true>(stage_rows32(a, tl, rows32, lane), tl.nm, a.A, F, J,
-                                                                   H, SLOT == 2 ? 0.5 : 0.0, a.alv, bt, lds, lane,
-                                                                   nullptr, 0, s);
+  const ListRows32 rows = stage_rows32<ORDER>(a, tl, rows32, lane);
+  gmf_tile<KIND, ORDER, OT, SLOT == SLOT_POW, ListRows32, 1, true>(rows, tl.nm, a.A, F, J, H, SLOT == 2 ? 0.5 : 0.0,
+                                                                   a.alv, bt, lds, lane, nullptr, 0, s);
 }
 
 template <int C, int C1, class OT>
